@@ -1,0 +1,195 @@
+"""Benchmark of the stencil path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+A step is one fused sweep (out += S(in), in_next = in + 1) over the whole
+grid, the per-iteration work of Cxx11/stencil.cc:183-190.  Default workload:
+FP64 star R=2 on a 30000 x 30000 grid with the reference's analytic init,
+100 timed iterations after 1 untimed warm-up sweep (the reference times
+iterations 1..K after sweep 0, Cxx11/stencil.cc:185).  The run is validated
+with the reference's closed form, norm = 2*(sweeps) within 1e-8.
+
+Prints ONE JSON line.  `value` is the algorithmic HBM bandwidth of the
+sweep (GB/s): B = 2*s*n^2 + 2*s*(n-2R)^2 bytes per sweep (read in, write
+in_next on every point; read and write out on the interior), inputs resident
+in HBM, timed with HIP events on the stream the sweeps run on.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "star-2 stencil GB/s & % HBM peak at n=30000; served solves/s + p99 latency"
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+WORKLOADS = {
+    # name: (pattern, radius, dtype, n)
+    "star2_f64_n30000": (0, 2, 0, 30000),
+    "grid2_f64_n30000": (1, 2, 0, 30000),
+    "star2_f32_n30000": (0, 2, 1, 30000),
+    "star2_bf16_n30000": (0, 2, 2, 30000),
+}
+DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
+
+
+def traffic_from_profiles(workload: str):
+    """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE, scaled by the copy-kernel calibration (profiles/)."""
+    p = ROOT / "profiles" / f"traffic_{workload}.txt"
+    if not p.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
+    """Own copy and triad kernels: the measured HBM peak on this GPU."""
+    s = torch.cuda.current_stream()
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    c = torch.empty_like(src).fill_(2.0)
+    res = {}
+    for name, fn, bytes_per in (
+            ("copy", lambda: K.check(K.lib().prk_copy(nbytes, dst.data_ptr(), src.data_ptr(), s.cuda_stream)),
+             2 * nbytes),
+            ("triad", lambda: K.check(K.lib().prk_triad(src.numel(), 3.0, dst.data_ptr(), src.data_ptr(),
+                                                         c.data_ptr(), s.cuda_stream)), 4 * nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        res[name] = bytes_per * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst, c
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(pattern: int, radius: int, n: int):
+    """The reference's OpenMP driver (Cxx11/stencil-openmp.cc, built from the
+    reference sources into oracle/_ref) on this host's cores; if it was not
+    built, the oracle's OpenMP restatement.  Bounded sample: 3 timed sweeps
+    (plus the warm-up sweep) of the same n x n grid."""
+    iters = 3
+    bytes_iter = 2 * 8 * n * n + 2 * 8 * (n - 2 * radius) ** 2
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    exe = ROOT / "oracle" / "_ref" / "stencil-openmp"
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    sample = f"{iters} timed + 1 warm-up sweeps, FP64 {'star' if pattern == 0 else 'grid'} R={radius}, n={n}"
+    if exe.exists():
+        r = subprocess.run([str(exe), str(iters), str(n), "32", "star" if pattern == 0 else "grid", str(radius)],
+                           capture_output=True, text=True, env=env, timeout=600)
+        m = re.search(r"Avg time \(s\):\s*([0-9.eE+-]+)", r.stdout)
+        t = re.search(r"Number of threads\s*=\s*(\d+)", r.stdout)
+        if r.returncode == 0 and m and "Solution validates" in r.stdout:
+            avg = float(m.group(1))
+            return {"value": bytes_iter / avg / 1e9, "unit": "GB/s", "cores": int(t.group(1)) if t else threads,
+                    "kind": "reference", "sample": sample, "avg_s_per_iter": avg,
+                    "mflops": float(re.search(r"Rate \(MFlops/s\):\s*([0-9.eE+-]+)", r.stdout).group(1))}
+    from tests._oracle import Oracle
+    avg, norm, team = Oracle().omp_run(pattern, radius, n, 32, iters, threads)
+    return {"value": bytes_iter / avg / 1e9, "unit": "GB/s", "cores": team, "kind": "port", "sample": sample,
+            "avg_s_per_iter": avg}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="star2_f64_n30000", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-peak", action="store_true", help="skip the copy/triad peak measurement")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU halo-exchange form "
+              "is not part of this build", file=sys.stderr)
+        return 2
+    if a.steps < 1 or a.warmup < 0:
+        print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+
+    import torch
+    import kernels_amd as K
+
+    pattern, radius, dtype, n = WORKLOADS[a.workload]
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    bufs = [K.alloc_grid(n, dtype), K.alloc_grid(n, dtype)]
+    out = K.alloc_grid(n, dtype)
+    K.init(K.INIT_ANALYTIC, 0, dtype, bufs[0], out, stream)
+    cur = 0
+    for _ in range(a.warmup):
+        K.sweep(pattern, radius, dtype, bufs[cur], bufs[cur ^ 1], out, stream)
+        cur ^= 1
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        K.sweep(pattern, radius, dtype, bufs[cur], bufs[cur ^ 1], out, stream)
+        cur ^= 1
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1)
+    sweeps = a.warmup + a.steps
+    norm = K.l1norm(dtype, radius, out, stream)
+    eps = {0: 1e-8, 1: 1e-4, 2: 1e-2}[dtype]
+    validated = abs(norm - 2.0 * sweeps) <= eps
+    del bufs, out
+    torch.cuda.empty_cache()
+
+    bytes_iter = K.stencil_bytes_per_iter(n, radius, dtype)
+    ms_step = ms / a.steps
+    gbs = bytes_iter / (ms_step * 1e-3) / 1e9
+    peak = {} if a.no_peak else measure_hbm_peak(torch, K)
+    res = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DT_NAME[dtype],
+        "data": "synthetic (reference analytic init in[i][j]=i+j, out=0)",
+        "config": {"workload": f"{'star' if pattern == 0 else 'grid'} R={radius} {DT_NAME[dtype]} "
+                               f"n={n} fused sweep (out+=S(in), in_next=in+1)",
+                   "pattern": "star" if pattern == 0 else "grid", "radius": radius, "n": n,
+                   "iterations": a.steps, "tile": 32},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "algorithmic_bytes_per_launch": bytes_iter},
+        "validated": bool(validated),
+        "norm": norm,
+        "mflops_reference_formula": round(K.nominal_flops(pattern, radius, n) / (ms_step * 1e-3) / 1e6, 1),
+        "points_per_s": (n - 2 * radius) ** 2 / (ms_step * 1e-3),
+        "wall_s_timed_region": round(t_wall, 4),
+    }
+    if peak:
+        res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
+        res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(pattern, radius, n) if dtype == 0 else None
+    print(json.dumps(res))
+    return 0 if validated else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

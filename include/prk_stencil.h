@@ -1,0 +1,150 @@
+/*
+ * prk_stencil.h -- C ABI of the MI355X (gfx950) stencil library
+ * (libprkstencil.so).
+ *
+ * The reference path is the Cxx11 stencil of the Parallel Research Kernels:
+ * a table of generated operators
+ *     void starR(const int n, const int t, const double* in, double* out)
+ * selected by (pattern, radius) (Cxx11/stencil.cc:142-159), the in += 1
+ * refresh (Cxx11/stencil.cc:189, HIP `add` Cxx11/stencil-hip.cc:75-83) and
+ * the L1-norm validation (Cxx11/stencil.cc:199-216).
+ *
+ * Conventions (all entry points):
+ *   - plain C types only; grid pointers are DEVICE pointers (hipMalloc'd or
+ *     any device-accessible allocation), row-major;
+ *   - every function returns 0 on success or a negative PRK_E* code, never
+ *     aborts; prk_last_error() gives the message of the calling thread's
+ *     last failure;
+ *   - `stream` arguments are hipStream_t values passed as void* (NULL = the
+ *     null stream); work is enqueued asynchronously unless stated otherwise;
+ *   - dtype: PRK_F64 (double), PRK_F32 (float), PRK_BF16 (raw bfloat16 bits,
+ *     uint16_t); the storage forms accumulate in FP32.
+ */
+#ifndef PRK_STENCIL_H
+#define PRK_STENCIL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRK_STENCIL_ABI_VERSION 1
+
+enum prk_pattern { PRK_STAR = 0, PRK_GRID = 1 };
+enum prk_dtype { PRK_F64 = 0, PRK_F32 = 1, PRK_BF16 = 2 };
+enum prk_init { PRK_INIT_ANALYTIC = 0, PRK_INIT_SEEDED = 1 };
+
+enum prk_status {
+    PRK_OK = 0,
+    PRK_EINVAL = -1,   /* bad argument (size, radius, pattern, dtype, alignment) */
+    PRK_ENOSTENCIL = -2, /* no operator for this (pattern, radius): the
+                            reference's `nothing` (Cxx11/stencil.cc:66-74) */
+    PRK_EHIP = -3,     /* HIP runtime error (launch, copy, allocation) */
+    PRK_ENOMEM = -4,   /* admission/budget refused or device allocation failed */
+    PRK_EBUSY = -5,    /* queue full / slot unavailable */
+    PRK_ENOTFOUND = -6,/* unknown request / handle */
+    PRK_ETIMEOUT = -7  /* a bounded device wait expired */
+};
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *prk_last_error(void);
+int prk_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Drop-in operators: the reference's generated table, one entry per
+ * (pattern, radius).  Replace
+ *   void starR(const int n, const int t, const double* RESTRICT in,
+ *              double* RESTRICT out)         Cxx11/stencil_openmp.hpp:18
+ *   __global__ void starR(const int n, const prk_float* in,
+ *              prk_float* out)               Cxx11/stencil_cuda.hpp:12
+ * Semantics: out[i*n+j] += S(in) on the interior R <= i,j < n-R, with the
+ * terms in the generator's order (Cxx11/generate-cxx-stencil.py:8-39), so
+ * results are bitwise those of Cxx11/stencil_seq.hpp.  in/out are n*n
+ * doubles, pitch n; the frame of out is not touched.  t is the reference's
+ * tile size: a tiling hint that never changes results (must be >= 1).
+ * Enqueued on the null stream, like the reference's HIP launches
+ * (Cxx11/stencil-hip.cc:213).
+ * ------------------------------------------------------------------------ */
+int prk_star1(int n, int t, const double *in, double *out);
+int prk_star2(int n, int t, const double *in, double *out);
+int prk_star3(int n, int t, const double *in, double *out);
+int prk_star4(int n, int t, const double *in, double *out);
+int prk_star5(int n, int t, const double *in, double *out);
+int prk_grid1(int n, int t, const double *in, double *out);
+int prk_grid2(int n, int t, const double *in, double *out);
+int prk_grid3(int n, int t, const double *in, double *out);
+int prk_grid4(int n, int t, const double *in, double *out);
+int prk_grid5(int n, int t, const double *in, double *out);
+
+/* The (pattern, radius) switch of Cxx11/stencil.cc:142-159 as one call.
+ * A radius outside 1..5 returns PRK_ENOSTENCIL (the reference aborts on the
+ * CPU and silently does nothing on HIP). */
+int prk_stencil(int pattern, int radius, int n, int t, const double *in, double *out);
+
+/* in[i*n+j] += 1 on all n*n points.  Replaces the HIP `add` kernel,
+ * Cxx11/stencil-hip.cc:75-83 (CPU: Cxx11/stencil.cc:189). Null stream. */
+int prk_add(int n, int t, double *in);
+
+/* ------------------------------------------------------------------------
+ * Fused sweep: out += S(in) on the interior and in_next = in + 1 on all
+ * n*n points, in one pass (in_next must not alias in).  Rows are `pitch`
+ * elements apart (pitch >= n); columns n..pitch-1 are never written.
+ * With pitch*size and the pointers 16-byte aligned the kernel moves whole
+ * 16-byte vectors; otherwise it falls back to element accesses.
+ * FP64 results are bitwise those of the unfused drop-in pair
+ * (prk_starR then prk_add).  Replaces the per-iteration pair of launches of
+ * Cxx11/stencil-hip.cc:213,216.
+ * ------------------------------------------------------------------------ */
+int prk_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
+              void *in_next, void *out, void *stream);
+
+/* Unfused sweep in any dtype: out += S(in) only. */
+int prk_stencil_ex(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
+                   void *out, void *stream);
+
+/* in += 1 on n*n points of any dtype (storage rounding for FP32/BF16). */
+int prk_add_ex(int dtype, int n, long long pitch, void *in, void *stream);
+
+/* Initial values (Cxx11/stencil.cc:170-181): PRK_INIT_ANALYTIC gives
+ * in[i][j] = i+j; PRK_INIT_SEEDED gives 2u-1 with u the top 53 bits of
+ * SplitMix64(seed, i*n+j) / 2^53.  out is zeroed (may be NULL).  Values are
+ * rounded once to the storage dtype (BF16 via FP32). */
+int prk_init(int init, uint64_t seed, int dtype, int n, long long pitch, void *in, void *out,
+             void *stream);
+
+/* L1 norm of the interior divided by (n-2R)^2 (Cxx11/stencil.cc:199-206),
+ * computed on the device in FP64 with compensated per-thread sums and a
+ * deterministic double-double cross-block reduction.  Synchronous: waits on
+ * `stream` and writes the result to *norm (host). */
+int prk_l1norm(int dtype, int n, long long pitch, int radius, const void *out, double *norm,
+               void *stream);
+
+/* The driver loop of Cxx11/stencil.cc:183-191 on the device: iterations+1
+ * fused sweeps over the ping-pong pair (buf0 holds the initial in; buf1 is
+ * scratch), sweep 0 untimed, sweeps 1..iterations timed with HIP events on
+ * `stream`.  *final_in_buf receives 0 or 1 (the buffer holding in after the
+ * last refresh), *avg_seconds the mean time of one timed sweep.
+ * Synchronous. */
+int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iterations,
+            void *buf0, void *buf1, void *out, void *stream, int *final_in_buf,
+            double *avg_seconds);
+
+/* Rows per wave the sweep would use for this size (tiling choice; exposed
+ * for the autotuner and tests).  0 = automatic. */
+int prk_set_rows_per_wave(int rows);
+
+/* ------------------------------------------------------------------------
+ * HBM calibration (nstream triad A += B + s*C, Cxx11/nstream-hip.cc:67-80;
+ * bytes 4*len*8 per call, :221) and a plain copy dst = src.
+ * ------------------------------------------------------------------------ */
+int prk_triad(long long len, double scalar, double *A, const double *B, const double *C,
+              void *stream);
+int prk_copy(long long bytes, void *dst, const void *src, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRK_STENCIL_H */

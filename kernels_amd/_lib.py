@@ -1,0 +1,71 @@
+"""ctypes binding of libprkstencil.so (C ABI: include/prk_stencil.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+STAR, GRID = 0, 1
+F64, F32, BF16 = 0, 1, 2
+INIT_ANALYTIC, INIT_SEEDED = 0, 1
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+c_int, c_ll, c_double, c_void_p, c_u64 = (ctypes.c_int, ctypes.c_longlong, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.c_uint64)
+c_int_p, c_double_p = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "prk_last_error": (ctypes.c_char_p, []),
+    "prk_abi_version": (c_int, []),
+    **{f"prk_{p}{r}": (c_int, [c_int, c_int, c_void_p, c_void_p])
+       for p in ("star", "grid") for r in range(1, 6)},
+    "prk_stencil": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "prk_add": (c_int, [c_int, c_int, c_void_p]),
+    "prk_sweep": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "prk_stencil_ex": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p]),
+    "prk_add_ex": (c_int, [c_int, c_int, c_ll, c_void_p, c_void_p]),
+    "prk_init": (c_int, [c_int, c_u64, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p]),
+    "prk_l1norm": (c_int, [c_int, c_int, c_ll, c_int, c_void_p, c_double_p, c_void_p]),
+    "prk_run": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_int, c_void_p, c_void_p, c_void_p,
+                        c_void_p, c_int_p, c_double_p]),
+    "prk_set_rows_per_wave": (c_int, [c_int]),
+    "prk_triad": (c_int, [c_ll, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "prk_copy": (c_int, [c_ll, c_void_p, c_void_p, c_void_p]),
+}
+
+
+class PrkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"prk error {code}: {msg}")
+        self.code = code
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("PRK_STENCIL_LIB", _HERE / "libprkstencil.so"))
+
+
+def lib() -> ctypes.CDLL:
+    """Load libprkstencil.so (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not p.exists():
+            raise FileNotFoundError(
+                f"{p} not found: build it with `python -m kernels_amd.build` (no CPU fallback exists)")
+        h = ctypes.CDLL(str(p))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise PrkError(rc, lib().prk_last_error().decode(errors="replace"))

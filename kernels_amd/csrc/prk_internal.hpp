@@ -1,0 +1,38 @@
+// Internal declarations shared by the translation units of libprkstencil.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "prk_common.hpp"
+#include "prk_stencil.h"
+
+namespace prk {
+
+// Thread-local last-error message behind prk_last_error().
+int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+int check_hip(hipError_t e, const char *what);
+
+// Geometry of one sweep launch.
+struct SweepPlan {
+    int dtype, pattern, radius, n;
+    long long pitch;
+    int rows_per_wave;
+    bool vec;
+};
+
+// Fills the launch geometry; rows_per_wave <= 0 picks automatically.
+int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
+               const void *in_next, const void *out, int rows_per_wave, SweepPlan *plan);
+
+// Enqueue one sweep (in_next == nullptr: unfused).
+int launch_sweep(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
+int launch_sweep_f64(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
+int launch_sweep_f32(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
+int launch_sweep_bf16(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
+
+int default_rows_per_wave();
+
+}  // namespace prk

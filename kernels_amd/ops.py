@@ -1,0 +1,116 @@
+"""Python face of the C ABI, with torch tensors as device buffers.
+
+Grids are 2-D tensors of shape (n, pitch) on a HIP device; pitch >= n is the
+row stride in elements.  BF16 storage uses torch.bfloat16 tensors (the
+library reads the raw bits).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import BF16, F32, F64, GRID, STAR, check, lib
+
+DTYPES = {"f64": F64, "f32": F32, "bf16": BF16}
+_TORCH = {F64: torch.float64, F32: torch.float32, BF16: torch.bfloat16}
+_ESIZE = {F64: 8, F32: 4, BF16: 2}
+
+# The reference's operator table (Cxx11/stencil.cc:142-159), by name.
+OPERATORS = {f"{p}{r}": (STAR if p == "star" else GRID, r) for p in ("star", "grid") for r in range(1, 6)}
+
+
+def torch_dtype(dtype: int) -> torch.dtype:
+    return _TORCH[dtype]
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _pitch(t: torch.Tensor) -> int:
+    return t.stride(0) if t.dim() == 2 else t.shape[-1]
+
+
+def alloc_grid(n: int, dtype: int = F64, device: str = "cuda", pitch: int | None = None,
+               fill: float | None = None) -> torch.Tensor:
+    """n x n grid stored with row stride `pitch` (default: n rounded up to a
+    16-byte multiple).  Returns the (n, n) view; its storage is (n, pitch)."""
+    es = _ESIZE[dtype]
+    if pitch is None:
+        per16 = 16 // es
+        pitch = (n + per16 - 1) // per16 * per16
+    base = torch.empty((n, pitch), dtype=_TORCH[dtype], device=device)
+    if fill is not None:
+        base.fill_(fill)
+    return base[:, :n]
+
+
+def stencil(name_or_pattern, n: int, t: int, inp: torch.Tensor, out: torch.Tensor, radius: int | None = None):
+    """Drop-in operator: out += S(in), FP64, pitch n, null stream."""
+    if isinstance(name_or_pattern, str):
+        pattern, radius = OPERATORS[name_or_pattern]
+        check(getattr(lib(), f"prk_{name_or_pattern}")(n, t, inp.data_ptr(), out.data_ptr()))
+    else:
+        check(lib().prk_stencil(int(name_or_pattern), int(radius), n, t, inp.data_ptr(), out.data_ptr()))
+
+
+def add(n: int, t: int, inp: torch.Tensor) -> None:
+    check(lib().prk_add(n, t, inp.data_ptr()))
+
+
+def sweep(pattern: int, radius: int, dtype: int, inp: torch.Tensor, in_next: torch.Tensor,
+          out: torch.Tensor, stream=None) -> None:
+    n = inp.shape[0]
+    check(lib().prk_sweep(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(in_next), _ptr(out),
+                          _stream(stream)))
+
+
+def stencil_ex(pattern: int, radius: int, dtype: int, inp: torch.Tensor, out: torch.Tensor, stream=None) -> None:
+    n = inp.shape[0]
+    check(lib().prk_stencil_ex(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(out), _stream(stream)))
+
+
+def init(kind: int, seed: int, dtype: int, inp: torch.Tensor, out: torch.Tensor | None, stream=None) -> None:
+    n = inp.shape[0]
+    check(lib().prk_init(kind, ctypes.c_uint64(seed), dtype, n, _pitch(inp), _ptr(inp), _ptr(out),
+                         _stream(stream)))
+
+
+def l1norm(dtype: int, radius: int, out: torch.Tensor, stream=None) -> float:
+    n = out.shape[0]
+    v = ctypes.c_double(0.0)
+    check(lib().prk_l1norm(dtype, n, _pitch(out), radius, _ptr(out), ctypes.byref(v), _stream(stream)))
+    return v.value
+
+
+def run(pattern: int, radius: int, dtype: int, iterations: int, buf0: torch.Tensor, buf1: torch.Tensor,
+        out: torch.Tensor, stream=None) -> tuple[int, float]:
+    """iterations+1 fused sweeps; returns (index of the buffer holding the
+    final in, average seconds per timed sweep)."""
+    n = buf0.shape[0]
+    fb = ctypes.c_int(0)
+    avg = ctypes.c_double(0.0)
+    check(lib().prk_run(pattern, radius, dtype, n, _pitch(buf0), iterations, _ptr(buf0), _ptr(buf1),
+                        _ptr(out), _stream(stream), ctypes.byref(fb), ctypes.byref(avg)))
+    return fb.value, avg.value
+
+
+def stencil_bytes_per_iter(n: int, radius: int, dtype: int) -> int:
+    """Algorithmic HBM bytes of one fused sweep: read in and write in_next
+    on all n^2 points, read and write out on the (n-2R)^2 interior."""
+    s = _ESIZE[dtype]
+    m = n - 2 * radius
+    return 2 * s * n * n + 2 * s * m * m
+
+
+def nominal_flops(pattern: int, radius: int, n: int) -> int:
+    """The reference's flop count per sweep (Cxx11/stencil.cc:221-222)."""
+    size = 4 * radius + 1 if pattern == STAR else (2 * radius + 1) ** 2
+    return (2 * size + 1) * (n - 2 * radius) ** 2

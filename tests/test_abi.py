@@ -1,0 +1,93 @@
+"""The C ABI without a GPU: the library loads, exports every function
+include/*.h declares, and rejects bad arguments with the documented codes
+before touching the device."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import kernels_amd
+from kernels_amd._lib import PrkError, check
+
+REPO = Path(__file__).resolve().parent.parent
+HEADERS = sorted((REPO / "include").glob("*.h"))
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(prk_\w+)\s*\(", re.M)
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        text = re.sub(r"#define.*", "", text)
+        names.update(DECL.findall(text))
+    return sorted(names)
+
+
+def test_headers_exist():
+    assert HEADERS, "no include/*.h"
+    assert len(declared_functions()) >= 20
+
+
+@pytest.mark.parametrize("name", declared_functions())
+def test_symbol_exported(name):
+    h = kernels_amd.lib()
+    assert hasattr(h, name), f"{name} declared but not exported"
+
+
+def test_symbols_exported_by_nm():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(kernels_amd.lib_path())],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (prk_\w+)", out))
+    missing = set(declared_functions()) - exported
+    assert not missing, missing
+
+
+def test_abi_version():
+    assert kernels_amd.lib().prk_abi_version() == 1
+
+
+def _rc(fn, *args):
+    return fn(*args)
+
+
+def test_bad_arguments_rejected_before_device():
+    L = kernels_amd.lib()
+    # radius outside the generated set: the reference's `nothing`
+    assert L.prk_stencil(0, 6, 100, 32, 16, 16) == -2
+    assert b"radius 6" in L.prk_last_error()
+    assert L.prk_stencil(0, 0, 100, 32, 16, 16) == -2
+    assert L.prk_stencil(2, 2, 100, 32, 16, 16) == -1       # unknown pattern
+    assert L.prk_stencil(0, 2, 4, 32, 16, 16) == -1         # 2R+1 > n
+    assert L.prk_stencil(0, 2, 100, 0, 16, 16) == -1        # tile < 1
+    assert L.prk_stencil(0, 2, 100, 32, None, 16) == -1     # null grid
+    assert L.prk_star2(0, 32, 16, 16) == -1                 # n < 1
+    assert L.prk_sweep(0, 2, 7, 100, 100, 16, 32, 48, None) == -1    # dtype
+    assert L.prk_sweep(0, 2, 0, 100, 99, 16, 32, 48, None) == -1     # pitch < n
+    assert L.prk_sweep(0, 2, 0, 100, 100, 16, 16, 48, None) == -1    # in_next aliases in
+    assert L.prk_sweep(0, 2, 0, 100, 100, 16, None, 48, None) == -1  # fused needs in_next
+    assert L.prk_run(0, 2, 0, 100, 100, 0, 16, 32, 48, None, None, None) == -1  # iterations < 1
+    assert L.prk_init(5, 0, 0, 10, 10, 16, 32, None) == -1
+    assert L.prk_set_rows_per_wave(-1) == -1
+    with pytest.raises(PrkError) as e:
+        check(L.prk_stencil(1, 9, 100, 32, 16, 16))
+    assert e.value.code == -2
+
+
+def test_cli_argument_errors():
+    exe = REPO / "kernels_amd" / "prk-stencil"
+    if not exe.exists():
+        pytest.skip("prk-stencil not built")
+    cases = [
+        ([], "Usage: <# iterations> <array dimension> [<tile_size> <star/grid> <radius>]"),
+        (["0", "100"], "ERROR: iterations must be >= 1"),
+        (["10", "0"], "ERROR: grid dimension must be positive"),
+        (["10", "46341"], "ERROR: grid dimension too large - overflow risk"),
+        (["10", "100", "32", "star", "0"], "ERROR: Stencil radius negative or too large"),
+        (["10", "4", "32", "star", "2"], "ERROR: Stencil radius negative or too large"),
+    ]
+    for args, msg in cases:
+        r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, (args, r.stdout)
+        assert msg in r.stdout, (args, r.stdout)

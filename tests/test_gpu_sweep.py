@@ -1,0 +1,173 @@
+"""GPU parity of the stencil path against the CPU oracle, through the C ABI.
+
+FP64 is bitwise: the kernels sum the taps in the generator's order with no
+contraction, exactly as the oracle (pinned to the reference's own generated
+operators in test_oracle.py).  FP32/BF16 storage forms are bitwise against
+the oracle's restatement of the same paired FP32 arithmetic, and their error
+against the FP64 oracle is bounded separately.
+"""
+import numpy as np
+import pytest
+import torch
+
+import kernels_amd as K
+from tests._oracle import BF16, F32, bf16_bits_to_f64
+
+pytestmark = pytest.mark.gpu
+OPS = [(p, r) for p in (0, 1) for r in range(1, 6)]
+NAME = {0: "star", 1: "grid"}
+
+
+def bits(a: np.ndarray) -> np.ndarray:
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32 if a.dtype == np.float32 else np.uint16)
+
+
+def to_dev(a: np.ndarray, gpu) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
+
+
+def to_host(t: torch.Tensor) -> np.ndarray:
+    torch.cuda.synchronize()
+    if t.dtype == torch.bfloat16:
+        return t.contiguous().view(torch.int16).cpu().numpy().view(np.uint16)
+    return t.contiguous().cpu().numpy()
+
+
+@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("n", [11, 64, 129, 333, 700])
+def test_dropin_operator_bitwise(gpu, oracle, pattern, radius, n):
+    if 2 * radius + 1 > n:
+        pytest.skip("radius too large")
+    print(f"dropin {NAME[pattern]}{radius} n={n}", flush=True)
+    rng = np.random.default_rng(n * 100 + radius * 10 + pattern)
+    x = rng.uniform(-1, 1, (n, n)) * 10.0
+    o = rng.uniform(-1, 1, (n, n))
+    xd, od = to_dev(x, gpu), to_dev(o, gpu)
+    K.stencil(f"{NAME[pattern]}{radius}", n, 32, xd, od)
+    oracle.stencil(pattern, radius, n, 32, x, o)
+    got = to_host(od)
+    assert np.array_equal(bits(got), bits(o)), np.abs(got - o).max()
+    assert np.array_equal(to_host(xd), x)  # in is read-only
+
+
+@pytest.mark.parametrize("n", [5, 64, 257, 1000])
+def test_dropin_add_bitwise(gpu, oracle, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1e3, 1e3, (n, n))
+    xd = to_dev(x, gpu)
+    K.add(n, 32, xd)
+    oracle.add(n, x)
+    assert np.array_equal(bits(to_host(xd)), bits(x))
+
+
+@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("n,pitch", [(64, 64), (100, 104), (129, 130), (515, 520), (37, 37)])
+def test_fused_sweep_equals_stencil_plus_add(gpu, oracle, pattern, radius, n, pitch):
+    if 2 * radius + 1 > n:
+        pytest.skip("radius too large")
+    print(f"fused {NAME[pattern]}{radius} n={n} pitch={pitch}", flush=True)
+    rng = np.random.default_rng(7 * n + radius)
+    x = rng.uniform(-1, 1, (n, n))
+    o = rng.uniform(-1, 1, (n, n))
+    xin = K.alloc_grid(n, K.F64, pitch=pitch, fill=float("nan"))
+    xnx = K.alloc_grid(n, K.F64, pitch=pitch, fill=-7.0)
+    od = K.alloc_grid(n, K.F64, pitch=pitch, fill=float("nan"))
+    xin.copy_(to_dev(x, gpu))
+    od.copy_(to_dev(o, gpu))
+    K.sweep(pattern, radius, K.F64, xin, xnx, od)
+    oracle.stencil(pattern, radius, n, 32, x, o)
+    assert np.array_equal(bits(to_host(od)), bits(o))
+    oracle.add(n, x)
+    assert np.array_equal(bits(to_host(xnx)), bits(x))
+    # padding columns n..pitch-1 are never written
+    if pitch > n:
+        pad = xnx.as_strided((n, pitch - n), (pitch, 1), xnx.storage_offset() + n)
+        assert torch.all(pad == -7.0)
+
+
+@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("init,seed", [(0, 0), (1, 42)])
+def test_run_bitwise(gpu, oracle, pattern, radius, init, seed):
+    n, iters = 150, 5
+    print(f"run {NAME[pattern]}{radius} init={init}", flush=True)
+    b0 = K.alloc_grid(n, K.F64)
+    b1 = K.alloc_grid(n, K.F64)
+    out = K.alloc_grid(n, K.F64)
+    K.init(init, seed, K.F64, b0, out)
+    fb, avg = K.run(pattern, radius, K.F64, iters, b0, b1, out)
+    ref_out, ref_in = oracle.run(pattern, radius, n, 32, iters, init, seed)
+    assert np.array_equal(bits(to_host(out)), bits(ref_out))
+    assert np.array_equal(bits(to_host((b0, b1)[fb])), bits(ref_in))
+    assert fb == (iters + 1) % 2 and avg > 0
+    norm = K.l1norm(K.F64, radius, out)
+    ref_norm = oracle.l1norm(n, radius, ref_out)
+    assert abs(norm - ref_norm) <= 1e-14 * abs(ref_norm)
+    if init == 0:
+        assert abs(norm - 2.0 * (iters + 1)) <= 1e-8
+
+
+def test_init_matches_oracle(gpu, oracle):
+    n = 77
+    for kind, seed in ((0, 0), (1, 5), (1, 2**63 + 11)):
+        a = K.alloc_grid(n, K.F64)
+        o = K.alloc_grid(n, K.F64, fill=3.0)
+        K.init(kind, seed, K.F64, a, o)
+        ra, ro = oracle.init(kind, seed, n)
+        assert np.array_equal(bits(to_host(a)), bits(ra))
+        assert np.all(to_host(o) == 0.0)
+
+
+def test_config1_on_gpu(gpu):
+    # 10 iterations, n=1000, tile 32, star 2: the reference's CPU-runnable case
+    n = 1000
+    b0, b1, out = K.alloc_grid(n), K.alloc_grid(n), K.alloc_grid(n)
+    K.init(K.INIT_ANALYTIC, 0, K.F64, b0, out)
+    K.run(K.STAR, 2, K.F64, 10, b0, b1, out)
+    assert K.l1norm(K.F64, 2, out) == 22.0
+    assert torch.all(out[2:-2, 2:-2] == 22.0)
+
+
+@pytest.mark.parametrize("dtype", [F32, BF16])
+@pytest.mark.parametrize("pattern,radius", OPS)
+def test_storage_forms_bitwise(gpu, oracle, dtype, pattern, radius):
+    n, iters = 97, 4
+    print(f"lp dtype={dtype} {NAME[pattern]}{radius}", flush=True)
+    b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+    K.init(K.INIT_SEEDED, 17, dtype, b0, out)
+    fb, _ = K.run(pattern, radius, dtype, iters, b0, b1, out)
+    ref_out, ref_in = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 17)
+    assert np.array_equal(to_host(out).view(ref_out.dtype), ref_out)
+    assert np.array_equal(to_host((b0, b1)[fb]).view(ref_in.dtype), ref_in)
+    norm = K.l1norm(dtype, radius, out)
+    assert abs(norm - oracle.l1norm_lp(dtype, n, radius, ref_out)) <= 1e-13 * abs(norm)
+
+
+@pytest.mark.parametrize("dtype,bound", [(F32, 2e-5), (BF16, 0.1)])
+def test_storage_form_error_vs_f64(gpu, oracle, dtype, bound):
+    n, iters, pattern, radius = 128, 10, 1, 2
+    b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+    K.init(K.INIT_SEEDED, 3, dtype, b0, out)
+    K.run(pattern, radius, dtype, iters, b0, b1, out)
+    ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 3)
+    got = to_host(out)
+    got = bf16_bits_to_f64(got) if dtype == BF16 else got.astype(np.float64)
+    inner = (slice(radius, n - radius),) * 2
+    err = np.abs(got[inner] - ref[inner])
+    normwise = np.linalg.norm(err) / np.linalg.norm(ref[inner])
+    print(f"dtype={dtype} normwise={normwise:.3e} max={err.max():.3e}")
+    assert normwise < bound
+
+
+def test_triad_and_copy(gpu):
+    n = 1 << 20
+    A = torch.full((n + 1,), 1.0, dtype=torch.float64, device=gpu)
+    B = torch.full((n + 1,), 2.0, dtype=torch.float64, device=gpu)
+    C = torch.full((n + 1,), 2.0, dtype=torch.float64, device=gpu)
+    for _ in range(3):
+        K.check(K.lib().prk_triad(n + 1, 3.0, A.data_ptr(), B.data_ptr(), C.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert torch.all(A == 1.0 + 3 * 8.0)
+    D = torch.empty_like(A)
+    K.check(K.lib().prk_copy(n * 8, D.data_ptr(), A.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert torch.equal(D[:n], A[:n])
