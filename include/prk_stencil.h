@@ -131,9 +131,16 @@ int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iter
             void *buf0, void *buf1, void *out, void *stream, int *final_in_buf,
             double *avg_seconds);
 
-/* Rows per wave the sweep would use for this size (tiling choice; exposed
- * for the autotuner and tests).  0 = automatic. */
+/* Tiling overrides for the autotuner and tests (process-wide; they never
+ * change results).  rows: row-block height per wavefront, 0 = automatic.
+ * variant: fused-sweep pipeline variant 0..5 (pipeline depth 1..3 x
+ * temporal / non-temporal streams), -1 = the tuned or default choice. */
 int prk_set_rows_per_wave(int rows);
+int prk_set_sweep_variant(int variant);
+/* order: workgroup-to-tile order (0 column-major runs per XCD, 1 row-major,
+ * 2 column ranges per XCD walked row-major); wgs_per_cu: cap on resident
+ * workgroups per CU (0 = none). */
+int prk_set_sweep_order(int order, int wgs_per_cu);
 
 /* ------------------------------------------------------------------------
  * HBM calibration (nstream triad A += B + s*C, Cxx11/nstream-hip.cc:67-80;

@@ -32,6 +32,8 @@ SIGNATURES = {
     "prk_run": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_int, c_void_p, c_void_p, c_void_p,
                         c_void_p, c_int_p, c_double_p]),
     "prk_set_rows_per_wave": (c_int, [c_int]),
+    "prk_set_sweep_variant": (c_int, [c_int]),
+    "prk_set_sweep_order": (c_int, [c_int, c_int]),
     "prk_triad": (c_int, [c_ll, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "prk_copy": (c_int, [c_ll, c_void_p, c_void_p, c_void_p]),
 }

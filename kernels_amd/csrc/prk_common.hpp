@@ -65,16 +65,21 @@ __host__ __device__ constexpr TapList make_taps(int pattern, int R)
     return L;
 }
 
-// Taps of positive weight, in generator order.  Every stencil here is
-// antisymmetric (W(-d) = -W(d)), so S(x)[p] = sum_k w_k (x[p+d_k] - x[p-d_k])
-// over these taps; the reduced-precision forms evaluate that paired form.
+// Taps of positive weight.  Every stencil here is antisymmetric
+// (W(-d) = -W(d)), so S(x)[p] = sum_k w_k (x[p+d_k] - x[p-d_k]) over these
+// taps; the reduced-precision forms evaluate that paired form.  Order: by
+// row-offset magnitude |di| = 0..R, generator order within one magnitude, so
+// a kernel needs only the two rows p±|di| at a time.
 __host__ __device__ constexpr TapList make_pairs(int pattern, int R)
 {
     const TapList all = make_taps(pattern, R);
     TapList P{};
     P.count = 0;
-    for (int k = 0; k < all.count; ++k)
-        if (all.t[k].w > 0.0) P.t[P.count++] = all.t[k];
+    for (int a = 0; a <= R; ++a)
+        for (int k = 0; k < all.count; ++k) {
+            const int ad = all.t[k].di < 0 ? -all.t[k].di : all.t[k].di;
+            if (all.t[k].w > 0.0 && ad == a) P.t[P.count++] = all.t[k];
+        }
     return P;
 }
 

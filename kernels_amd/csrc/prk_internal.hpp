@@ -20,8 +20,17 @@ struct SweepPlan {
     int dtype, pattern, radius, n;
     long long pitch;
     int rows_per_wave;
-    bool vec;
+    int mode;           // MODE_ELEM / MODE_VEC / MODE_VEC_LD (sweep.hpp)
+    int variant;        // pipeline depth / cache policy of the fused vector sweep
+    bool pad_writable;  // columns n..pitch-1 may be overwritten
+    int order;          // workgroup -> tile order (sweep.hpp tile_of)
+    int wgs_per_cu;     // occupancy cap, 0 = none
 };
+
+// Number of fused-sweep variants (pipeline depth 1..3 x temporal/non-temporal).
+constexpr int kSweepVariants = 6;
+// Default variant for a (dtype, pattern, radius) before any tuning.
+int default_variant(int dtype, int pattern, int radius);
 
 // Fills the launch geometry; rows_per_wave <= 0 picks automatically.
 int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,

@@ -33,9 +33,17 @@ namespace prk {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// Build-time tuning knobs (defaults are the measured best on MI355X).
+#ifndef PRK_SWEEP_PREFETCH
+#define PRK_SWEEP_PREFETCH 2  // rows in the register pipeline
+#endif
+#ifndef PRK_SWEEP_WAVES_PER_EU
+#define PRK_SWEEP_WAVES_PER_EU 0  // >0: ask the compiler for this occupancy
+#endif
+
 constexpr int kWavesPerBlock = 4;
 constexpr int kThreads = 64 * kWavesPerBlock;
-constexpr int kPrefetch = 3;  // rows in the register pipeline
+constexpr int kPrefetch = PRK_SWEEP_PREFETCH;
 
 struct SweepArgs {
     const void *in;
@@ -48,8 +56,39 @@ struct SweepArgs {
     int ngroups;        // ceil(nstrips / 4)
     int nrowblocks;     // ceil(n / rows_per_wave)
     int ntiles;         // ngroups * nrowblocks
-    int chunk;          // ceil(ntiles / 8): tiles per XCD
+    int chunk;          // tiles per XCD (order 0) / column groups per XCD (order 2)
+    int pad_writable;   // columns n..pitch-1 may be overwritten
+    int order;          // tile order, see tile_of()
 };
+
+// Workgroup -> tile.  Workgroup b runs on XCD b%8 (round-robin dispatch).
+//   order 0: each XCD takes a contiguous run of tiles in column-major order
+//            (a column group top to bottom, then the next);
+//   order 1: row-major over the whole grid, no XCD grouping;
+//   order 2: each XCD owns a contiguous range of column groups and walks it
+//            row-major, so the workgroups running at once on an XCD cover a
+//            narrow band of rows and a tile's vertical neighbour (whose rows
+//            it re-reads as halo) runs on the same XCD shortly after it.
+__device__ inline bool tile_of(const SweepArgs &a, int b, int &group, int &rb)
+{
+    if (a.order == 1) {
+        if (b >= a.ntiles) return false;
+        rb = b / a.ngroups;
+        group = b - rb * a.ngroups;
+        return true;
+    }
+    const int x = b & 7, k = b >> 3;
+    if (a.order == 2) {
+        rb = k / a.chunk;
+        group = x * a.chunk + (k - rb * a.chunk);
+        return rb < a.nrowblocks && (k - rb * a.chunk) < a.chunk && group < a.ngroups;
+    }
+    const int tile = x * a.chunk + k;
+    if (k >= a.chunk || tile >= a.ntiles) return false;
+    group = tile / a.nrowblocks;
+    rb = tile - group * a.nrowblocks;
+    return true;
+}
 
 template <int DT, int R>
 struct SweepGeom {
@@ -114,18 +153,35 @@ __device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int
     return s;
 }
 
-// FP32 paired sum: acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs.
-template <typename T, int P, int R, size_t... I>
-__device__ inline float pair_sum_f32(const T *const (&rows)[2 * R + 1], int v,
+// Reduced-precision forms.  Each row a lane reads is fetched from LDS as whole
+// 16-byte vectors covering columns [-HP, V+HP) around its own V columns and
+// widened to float once; the paired sum then reads that register window.
+template <typename T, int V, int HP>
+__device__ inline void load_window(const T *row, float (&w)[V + 2 * HP])
+{
+    constexpr int NV = (V + 2 * HP) / V;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        union { u32x4 v; T e[V]; } x;
+        x.v = *(const u32x4 *)(row + (q * V - HP));
+#pragma unroll
+        for (int e = 0; e < V; ++e) w[q * V + e] = to_f32<T>(x.e[e]);
+    }
+}
+
+// acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs (pair order of
+// make_pairs: by row-offset magnitude).
+template <int P, int R, int V, int HP, size_t... I>
+__device__ inline float pair_sum_f32(const float (&win)[2 * R + 1][V + 2 * HP], int v,
                                      std::index_sequence<I...>)
 {
 #pragma clang fp contract(off)
     constexpr const TapList &L = Stencil<P, R>::pairs;
-    float acc = (float)L.t[0].w * (to_f32<T>(rows[R + L.t[0].di][v + L.t[0].dj]) -
-                                   to_f32<T>(rows[R - L.t[0].di][v - L.t[0].dj]));
+    float acc = (float)L.t[0].w * (win[R + L.t[0].di][HP + v + L.t[0].dj] -
+                                   win[R - L.t[0].di][HP + v - L.t[0].dj]);
     ((acc = __builtin_fmaf((float)L.t[I + 1].w,
-                           to_f32<T>(rows[R + L.t[I + 1].di][v + L.t[I + 1].dj]) -
-                               to_f32<T>(rows[R - L.t[I + 1].di][v - L.t[I + 1].dj]),
+                           win[R + L.t[I + 1].di][HP + v + L.t[I + 1].dj] -
+                               win[R - L.t[I + 1].di][HP + v - L.t[I + 1].dj],
                            acc)),
      ...);
     return acc;
@@ -133,7 +189,7 @@ __device__ inline float pair_sum_f32(const T *const (&rows)[2 * R + 1], int v,
 
 // The V outputs of one lane for output row o.  rows[k] points at the LDS row
 // holding input row o-R+k, already offset to this lane's first column.
-template <int DT, int P, int R, int V>
+template <int DT, int P, int R, int V, int HP>
 __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (&rows)[2 * R + 1],
                                       const typename DTypeTraits<DT>::T *outv,
                                       typename DTypeTraits<DT>::T *res)
@@ -146,29 +202,93 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
         for (int v = 0; v < V; ++v)
             res[v] = outv[v] + tap_sum_f64<P, R>(rows, v, std::make_index_sequence<St::taps.count - 1>{});
     } else {
+        float win[2 * R + 1][V + 2 * HP];
+#pragma unroll
+        for (int k = 0; k < 2 * R + 1; ++k) load_window<T, V, HP>(rows[k], win[k]);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            const float acc = pair_sum_f32<T, P, R>(rows, v, std::make_index_sequence<St::pairs.count - 1>{});
+            const float acc = pair_sum_f32<P, R, V, HP>(win, v, std::make_index_sequence<St::pairs.count - 1>{});
             res[v] = from_f32<T>(to_f32<T>(outv[v]) + acc);
         }
     }
 }
 
-template <int DT, int P, int R, bool FUSE, bool VEC>
-__global__ __launch_bounds__(kThreads) void sweep_kernel(SweepArgs a)
+// ---------------------------------------------------------------------------
+// Raw buffer access.  Every HBM access of the sweep is a buffer instruction on
+// a per-row resource: an offset at or beyond the resource's byte range reads
+// zero and drops the write without touching memory.  Invalid rows, columns
+// and pipeline slots are therefore expressed as out-of-range offsets rather
+// than branches, which keeps the row loop straight-line code and lets the
+// compiler wait only for the loads a row actually needs.
+// ---------------------------------------------------------------------------
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+constexpr unsigned kOOB = 0x80000000u;         // offset beyond every row resource
+constexpr int kRsrcWord3 = 0x00020000;          // gfx9 raw buffer, 32-bit data format
+
+__device__ inline __amdgpu_buffer_rsrc_t row_rsrc(const void *row, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(row), (short)0, (int)bytes, kRsrcWord3);
+}
+
+template <int AUX>
+__device__ inline u32x4 bload16(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, AUX));
+}
+
+template <int AUX>
+__device__ inline void bstore16(__amdgpu_buffer_rsrc_t r, unsigned off, u32x4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, v), r, (int)off, 0, AUX);
+}
+
+// One element of storage type T.
+template <typename T, int AUX>
+__device__ inline T bload_elem(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, AUX));
+    else if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, AUX));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, AUX));
+}
+
+template <typename T, int AUX>
+__device__ inline void bstore_elem(__amdgpu_buffer_rsrc_t r, unsigned off, T v)
+{
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, v), r, (int)off, 0, AUX);
+    else if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)off, 0, AUX);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(short, v), r, (int)off, 0, AUX);
+}
+
+// Access modes of the sweep.
+//   MODE_VEC    rows 16-byte aligned and every written vector whole: n % V == 0,
+//               or the caller lets columns n..pitch-1 be overwritten.
+//   MODE_VEC_LD 16-byte vector loads, element stores (aligned rows, ragged n).
+//   MODE_ELEM   element loads and stores (rows not 16-byte aligned).
+enum { MODE_ELEM = 0, MODE_VEC = 1, MODE_VEC_LD = 2 };
+
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+__global__ __launch_bounds__(kThreads)
+#if PRK_SWEEP_WAVES_PER_EU > 0
+__attribute__((amdgpu_waves_per_eu(PRK_SWEEP_WAVES_PER_EU)))
+#endif
+void sweep_kernel(SweepArgs a)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
     constexpr int V = G::V, SW = G::SW, HP = G::HP, NH = G::NH, LR = G::LR, S = G::S;
+    constexpr int ES = (int)sizeof(T);
+    constexpr int AUX_STREAM = (NT & 1) ? 2 : 0;  // nt bit on out loads and all stores
     __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][S][LR];
 
-    // XCD-aware tile order: workgroup b runs on XCD b%8; give each XCD a
-    // contiguous run of tiles, tiles column-major (row blocks fastest).
-    const int b = blockIdx.x;
-    const int tile = (b & 7) * a.chunk + (b >> 3);
-    if ((b >> 3) >= a.chunk || tile >= a.ntiles) return;
-    const int group = tile / a.nrowblocks;
-    const int rb = tile - group * a.nrowblocks;
+    int group, rb;
+    if (!tile_of(a, blockIdx.x, group, rb)) return;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int strip = group * kWavesPerBlock + wave;
@@ -185,102 +305,110 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(SweepArgs a)
     const T *in = (const T *)a.in;
     T *in_next = (T *)a.in_next;
     T *out = (T *)a.out;
+    // Loads may read a whole padded row; stores stay inside [0, n) unless the
+    // caller allows the padding to be written.
+    const unsigned ld_bytes = (unsigned)(pitch * ES);
+    const unsigned st_bytes = a.pad_writable ? ld_bytes : (unsigned)(n * ES);
 
     const int cm = c0 + lane * V;  // first column of this lane's main vector
-    // Halo vector of this lane (lanes 0..NH-1 left, NH..2NH-1 right).
-    const bool has_halo = lane < 2 * NH;
+    // Halo vector of this lane (lanes 0..NH-1 left, NH..2NH-1 right); other
+    // lanes point their halo load out of range.
     const int ch = lane < NH ? c0 - HP + lane * V : c0 + SW + (lane - NH) * V;
     const int lh = lane < NH ? lane * V : HP + SW + (lane - NH) * V;  // LDS column
-    const bool main_ok = cm < n;
-    const bool halo_ok = has_halo && ch >= 0 && ch < n;
+    const unsigned off_main = (unsigned)(cm * ES);
+    const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;  // ch<0 wraps out of range
+    // Per-column write-back mask for out: frame columns keep their value.
+    bool interior[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) interior[v] = (cm + v >= R) && (cm + v < n - R);
 
-    auto load_row = [&](RowRegs<T, V> &r, int ri) {
-        const T *row = in + (long long)ri * pitch;
-        if constexpr (VEC) {
-            r.main.v = main_ok ? *(const u32x4 *)(row + cm) : u32x4{0, 0, 0, 0};
-            r.halo.v = halo_ok ? *(const u32x4 *)(row + ch) : u32x4{0, 0, 0, 0};
+    T(*ring)[LR] = lds[wave];
+
+    // Row pipeline: buf[k] holds input row (main + halo) and the out row R
+    // above it, loaded PF steps ahead.
+    RowRegs<T, V> buf[PF];
+    auto issue = [&](RowRegs<T, V> &r, int ri) {
+        const bool row_ok = ri < hi;
+        const int rr = row_ok ? ri : lo;
+        const __amdgpu_buffer_rsrc_t rin = row_rsrc(in + (long long)rr * pitch, ld_bytes);
+        const int o = ri - R;
+        const bool out_ok = o >= olo && o < ohi;
+        const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, ld_bytes);
+        if constexpr (MODE != MODE_ELEM) {
+            r.main.v = bload16<0>(rin, row_ok ? off_main : kOOB);
+            r.halo.v = bload16<0>(rin, row_ok ? off_halo : kOOB);
+            r.outv.v = bload16<AUX_STREAM>(rout, out_ok ? off_main : kOOB);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                r.main.e[v] = (cm + v < n) ? row[cm + v] : T(0);
-                r.halo.e[v] = (has_halo && ch + v >= 0 && ch + v < n) ? row[ch + v] : T(0);
-            }
-        }
-        const int o = ri - R;
-        if (o >= olo && o < ohi) {
-            const T *orow = out + (long long)o * pitch;
-            if constexpr (VEC) {
-                r.outv.v = main_ok ? *(const u32x4 *)(orow + cm) : u32x4{0, 0, 0, 0};
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; ++v) r.outv.e[v] = (cm + v < n) ? orow[cm + v] : T(0);
+                r.main.e[v] = bload_elem<T, 0>(rin, row_ok ? off_main + v * ES : kOOB);
+                r.halo.e[v] = bload_elem<T, 0>(rin, (row_ok && off_halo != kOOB) ? off_halo + v * ES : kOOB);
+                r.outv.e[v] = bload_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB);
             }
         }
     };
 
-    T(*ring)[LR] = lds[wave];
-    RowRegs<T, V> buf[kPrefetch];
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k)
-        if (lo + k < hi) load_row(buf[k], lo + k);
+    for (int k = 0; k < PF; ++k) issue(buf[k], lo + k);
 
     int slot = 0;  // ring slot of row ri: (ri - lo) % S
-    for (int base = lo; base < hi; base += kPrefetch) {
+    for (int base = lo; base < hi; base += PF) {
 #pragma unroll
-        for (int k = 0; k < kPrefetch; ++k) {
+        for (int k = 0; k < PF; ++k) {
             const int ri = base + k;
-            if (ri < hi) {
-                RowRegs<T, V> &r = buf[k];
-                // park row ri in the ring
-                *(u32x4 *)&ring[slot][HP + lane * V] = r.main.v;
-                if (has_halo) *(u32x4 *)&ring[slot][lh] = r.halo.v;
+            RowRegs<T, V> cur = buf[k];
+            issue(buf[k], ri + PF);
 
-                if (FUSE && ri >= r0 && ri < r1) {
-                    union { u32x4 v; T e[V]; } nx;
-                    plus_one<DT, V>(r.main.e, nx.e);
-                    T *dst = in_next + (long long)ri * pitch + cm;
-                    if (VEC && cm + V <= n) {
-                        *(u32x4 *)dst = nx.v;
-                    } else {
+            // park row ri in the ring
+            *(u32x4 *)&ring[slot][HP + lane * V] = cur.main.v;
+            if (lane < 2 * NH) *(u32x4 *)&ring[slot][lh] = cur.halo.v;
+
+            if constexpr (FUSE) {
+                const bool nx_ok = ri >= r0 && ri < r1;
+                const __amdgpu_buffer_rsrc_t rnx =
+                    row_rsrc(in_next + (long long)(nx_ok ? ri : r0) * pitch, st_bytes);
+                union { u32x4 v; T e[V]; } nx;
+                plus_one<DT, V>(cur.main.e, nx.e);
+                if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rnx, nx_ok ? off_main : kOOB, nx.v);
+                } else {
 #pragma unroll
-                        for (int v = 0; v < V; ++v)
-                            if (cm + v < n) dst[v] = nx.e[v];
-                    }
+                    for (int v = 0; v < V; ++v)
+                        bstore_elem<T, AUX_STREAM>(rnx, nx_ok ? off_main + v * ES : kOOB, nx.e[v]);
                 }
-
-                const int o = ri - R;
-                const bool do_out = (o >= olo && o < ohi);
-                RowRegs<T, V> cur = r;
-                if (ri + kPrefetch < hi) load_row(r, ri + kPrefetch);
-
-                // LDS writes of this wave land before its reads (in-order
-                // per wave); the fences keep the compiler from moving them.
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-                if (do_out) {
-                    // rows o-R .. o+R live in slots slot+1 .. slot+2R+1 (mod S)
-                    const T *rows[S];
-#pragma unroll
-                    for (int k2 = 0; k2 < S; ++k2) {
-                        int s2 = slot + 1 + k2;
-                        s2 = s2 >= S ? s2 - S : s2;
-                        rows[k2] = &ring[s2][HP + lane * V];
-                    }
-                    union { u32x4 v; T e[V]; } res;
-                    compute_points<DT, P, R, V>(rows, cur.outv.e, res.e);
-                    T *dst = out + (long long)o * pitch + cm;
-                    if (VEC && cm >= R && cm + V <= n - R) {
-                        *(u32x4 *)dst = res.v;
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < V; ++v)
-                            if (cm + v >= R && cm + v < n - R) dst[v] = res.e[v];
-                    }
-                }
-                slot = (slot + 1 == S) ? 0 : slot + 1;
             }
+
+            // LDS writes of this wave land before its reads (in-order per
+            // wave); the fences keep the compiler from moving them.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+            // Output row o = ri - R: rows o-R .. o+R live in slots
+            // slot+1 .. slot+2R+1 (mod S).  Rows outside the interior are
+            // computed too and their store goes out of range.
+            const int o = ri - R;
+            const bool out_ok = o >= olo && o < ohi;
+            const T *rows[S];
+#pragma unroll
+            for (int k2 = 0; k2 < S; ++k2) {
+                int s2 = slot + 1 + k2;
+                s2 = s2 >= S ? s2 - S : s2;
+                rows[k2] = &ring[s2][HP + lane * V];
+            }
+            union { u32x4 v; T e[V]; } res;
+            compute_points<DT, P, R, V, HP>(rows, cur.outv.e, res.e);
+#pragma unroll
+            for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : cur.outv.e[v];
+            const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, st_bytes);
+            if constexpr (MODE == MODE_VEC) {
+                bstore16<AUX_STREAM>(rout, out_ok ? off_main : kOOB, res.v);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    bstore_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB, res.e[v]);
+            }
+            slot = (slot + 1 == S) ? 0 : slot + 1;
         }
     }
 }

@@ -6,7 +6,7 @@
 
 namespace prk {
 
-template <int DT, int P, int R, bool FUSE, bool VEC>
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
 int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s)
 {
     typedef SweepGeom<DT, R> G;
@@ -21,21 +21,52 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
     a.ngroups = (a.nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
     a.nrowblocks = (p.n + p.rows_per_wave - 1) / p.rows_per_wave;
     a.ntiles = a.ngroups * a.nrowblocks;
-    a.chunk = (a.ntiles + 7) / 8;
-    const unsigned blocks = 8u * (unsigned)a.chunk;
-    hipLaunchKernelGGL((sweep_kernel<DT, P, R, FUSE, VEC>), dim3(blocks), dim3(kThreads), 0, s, a);
+    a.pad_writable = p.pad_writable ? 1 : 0;
+    a.order = p.order;
+    unsigned blocks;
+    if (p.order == 1) {
+        a.chunk = 0;
+        blocks = (unsigned)a.ntiles;
+    } else if (p.order == 2) {
+        a.chunk = (a.ngroups + 7) / 8;
+        blocks = 8u * (unsigned)a.chunk * (unsigned)a.nrowblocks;
+    } else {
+        a.chunk = (a.ntiles + 7) / 8;
+        blocks = 8u * (unsigned)a.chunk;
+    }
+    // Extra dynamic LDS caps the workgroups resident per CU (occupancy knob).
+    const size_t lds_cap = p.wgs_per_cu > 0 ? (160u * 1024u) / (unsigned)p.wgs_per_cu : 0;
+    const size_t dyn = lds_cap > G::lds_bytes ? lds_cap - G::lds_bytes : 0;
+    hipLaunchKernelGGL((sweep_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
     return check_hip(hipGetLastError(), "sweep launch");
 }
 
+// The fused, whole-vector sweep is the hot path: it is instantiated for
+// every pipeline depth / cache policy the tuner chooses from.  The other
+// forms use the default.
 template <int DT, int P, int R>
 int launch_pr(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s)
 {
     const bool fuse = in_next != nullptr;
-    if (p.vec)
-        return fuse ? launch_one<DT, P, R, true, true>(p, in, in_next, out, s)
-                    : launch_one<DT, P, R, false, true>(p, in, in_next, out, s);
-    return fuse ? launch_one<DT, P, R, true, false>(p, in, in_next, out, s)
-                : launch_one<DT, P, R, false, false>(p, in, in_next, out, s);
+    if (fuse && p.mode == MODE_VEC) {
+        const int v = p.variant;
+        switch (v) {
+        case 0: return launch_one<DT, P, R, true, MODE_VEC, 1, 0>(p, in, in_next, out, s);
+        case 1: return launch_one<DT, P, R, true, MODE_VEC, 2, 0>(p, in, in_next, out, s);
+        case 2: return launch_one<DT, P, R, true, MODE_VEC, 3, 0>(p, in, in_next, out, s);
+        case 3: return launch_one<DT, P, R, true, MODE_VEC, 1, 1>(p, in, in_next, out, s);
+        case 4: return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
+        case 5: return launch_one<DT, P, R, true, MODE_VEC, 3, 1>(p, in, in_next, out, s);
+        }
+        return set_error(PRK_EINVAL, "unknown sweep variant %d", v);
+    }
+    if (fuse) {
+        if (p.mode == MODE_VEC_LD) return launch_one<DT, P, R, true, MODE_VEC_LD, kPrefetch, 0>(p, in, in_next, out, s);
+        return launch_one<DT, P, R, true, MODE_ELEM, kPrefetch, 0>(p, in, in_next, out, s);
+    }
+    if (p.mode == MODE_VEC) return launch_one<DT, P, R, false, MODE_VEC, kPrefetch, 0>(p, in, in_next, out, s);
+    if (p.mode == MODE_VEC_LD) return launch_one<DT, P, R, false, MODE_VEC_LD, kPrefetch, 0>(p, in, in_next, out, s);
+    return launch_one<DT, P, R, false, MODE_ELEM, kPrefetch, 0>(p, in, in_next, out, s);
 }
 
 template <int DT>

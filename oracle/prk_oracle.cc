@@ -184,13 +184,16 @@ template <> inline uint16_t st<uint16_t>(float x) { return f32_to_bf16(x); }
 
 struct Pair { int di, dj; float w; };
 
-// Taps of positive weight in generator order, weights rounded to float.
+// Taps of positive weight, weights rounded to float, ordered by row-offset
+// magnitude |di| = 0..R and in generator order within one magnitude.
 std::vector<Pair> make_pairs(int pattern, int R)
 {
     const TapList L = make_taps(pattern, R);
     std::vector<Pair> v;
-    for (int k = 0; k < L.count; ++k)
-        if (L.t[k].w > 0.0) v.push_back(Pair{L.t[k].di, L.t[k].dj, (float)L.t[k].w});
+    for (int a = 0; a <= R; ++a)
+        for (int k = 0; k < L.count; ++k)
+            if (L.t[k].w > 0.0 && std::abs(L.t[k].di) == a)
+                v.push_back(Pair{L.t[k].di, L.t[k].dj, (float)L.t[k].w});
     return v;
 }
 

@@ -79,8 +79,9 @@ int prko_band(int pattern, int radius, int n, int iterations, int init,
 /* The reduced-precision storage forms, storage dtype PRKO_F32 or PRKO_BF16
  * (raw uint16 bit patterns), FP32 accumulation.  Arithmetic per point:
  *   acc = w1*(x[p+d1]-x[p-d1]); acc = fmaf(wk, x[p+dk]-x[p-dk], acc) ...
- * over the taps of positive weight in generator order (every stencil is
- * antisymmetric, W(-d) = -W(d)), then out = T(float(out) + acc) and
+ * over the taps of positive weight (every stencil is antisymmetric,
+ * W(-d) = -W(d)), ordered by row-offset magnitude |di| = 0..R and in
+ * generator order within one magnitude, then out = T(float(out) + acc) and
  * in = T(float(in) + 1).  BF16 rounding is round-to-nearest-even.
  * Initial values: T(float(i+j)) or T(float(seeded double)). */
 int prko_run_lp(int dtype, int pattern, int radius, int n, int iterations,
