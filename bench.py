@@ -44,7 +44,7 @@ DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 def traffic_from_profiles(workload: str):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE, scaled by the copy-kernel calibration (profiles/)."""
-    p = ROOT / "profiles" / f"traffic_{workload}.txt"
+    p = ROOT / "profiles" / f"sweep_{workload}.md"
     if not p.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
@@ -112,6 +112,12 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="star2_f64_n30000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-peak", action="store_true", help="skip the copy/triad peak measurement")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (use the built-in default tiling)")
+    ap.add_argument("--tuning-file", default=None,
+                    help="save the autotuned table here (or, with --no-tune, load it from here)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the sweeps, run one 2 GiB copy (known bytes) for counter calibration")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU halo-exchange form "
@@ -127,6 +133,17 @@ def main(argv=None) -> int:
     pattern, radius, dtype, n = WORKLOADS[a.workload]
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    tuning = K.Tuning()
+    t_tune = time.perf_counter()
+    if a.tuning_file and a.no_tune and os.path.exists(a.tuning_file):
+        K.lib().prk_tuning_load(a.tuning_file.encode())
+    if not a.no_tune:
+        K.check(K.lib().prk_autotune(pattern, radius, dtype, n, 2, tuning))
+        if a.tuning_file:
+            K.check(K.lib().prk_tuning_save(a.tuning_file.encode()))
+    else:
+        K.check(K.lib().prk_tuning_get(pattern, radius, dtype, n, tuning))
+    t_tune = time.perf_counter() - t_tune
     bufs = [K.alloc_grid(n, dtype), K.alloc_grid(n, dtype)]
     out = K.alloc_grid(n, dtype)
     K.init(K.INIT_ANALYTIC, 0, dtype, bufs[0], out, stream)
@@ -151,6 +168,14 @@ def main(argv=None) -> int:
     validated = abs(norm - 2.0 * sweeps) <= eps
     del bufs, out
     torch.cuda.empty_cache()
+    if a.copy_calibration:
+        nb = 2 << 30
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        K.check(K.lib().prk_copy(nb, dst.data_ptr(), src.data_ptr(), stream.cuda_stream))
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
 
     bytes_iter = K.stencil_bytes_per_iter(n, radius, dtype)
     ms_step = ms / a.steps
@@ -176,6 +201,7 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
                      "algorithmic_bytes_per_launch": bytes_iter},
+        "tuning": {**tuning.as_dict(), "autotuned": not a.no_tune, "tune_s": round(t_tune, 2)},
         "validated": bool(validated),
         "norm": norm,
         "mflops_reference_formula": round(K.nominal_flops(pattern, radius, n) / (ms_step * 1e-3) / 1e6, 1),

@@ -138,9 +138,37 @@ int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iter
 int prk_set_rows_per_wave(int rows);
 int prk_set_sweep_variant(int variant);
 /* order: workgroup-to-tile order (0 column-major runs per XCD, 1 row-major,
- * 2 column ranges per XCD walked row-major); wgs_per_cu: cap on resident
- * workgroups per CU (0 = none). */
+ * 2 column ranges per XCD walked row-major, 3 row-major with the column
+ * groups padded to a multiple of the 8 XCDs); wgs_per_cu: cap on resident
+ * workgroups per CU (0 = none).  -1 for either: the tuned choice. */
 int prk_set_sweep_order(int order, int wgs_per_cu);
+
+/* ------------------------------------------------------------------------
+ * Autotuner.  A tiling of the fused sweep: pipeline variant, row-block
+ * height per wavefront, resident workgroups per CU (0 = no cap) and tile
+ * order; gbs is the algorithmic bandwidth it measured (0 if not measured).
+ * prk_autotune times every candidate for (pattern, radius, dtype, n) on the
+ * current device with `reps` sweeps each (it allocates three n x n scratch
+ * grids), records the fastest in a process-wide table that later sweeps of
+ * that shape use, and returns it.  prk_tuning_get returns what a sweep of
+ * that shape would use now (tuned entry, nearest tuned size within 1.5x, or
+ * the built-in default).  Save/load keep the table in a text file;
+ * prk_tuning_load returns the number of entries read.
+ * ------------------------------------------------------------------------ */
+typedef struct prk_tuning {
+    int variant;
+    int rows_per_wave;
+    int wgs_per_cu;
+    int order;
+    double gbs;
+} prk_tuning;
+
+int prk_autotune(int pattern, int radius, int dtype, int n, int reps, prk_tuning *best);
+int prk_tuning_get(int pattern, int radius, int dtype, int n, prk_tuning *t);
+int prk_tuning_set(int pattern, int radius, int dtype, int n, const prk_tuning *t);
+int prk_tuning_clear(void);
+int prk_tuning_save(const char *path);
+int prk_tuning_load(const char *path);
 
 /* ------------------------------------------------------------------------
  * HBM calibration (nstream triad A += B + s*C, Cxx11/nstream-hip.cc:67-80;

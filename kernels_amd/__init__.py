@@ -9,7 +9,7 @@ is missing every call raises.
 from __future__ import annotations
 
 from ._lib import (  # noqa: F401
-    BF16, F32, F64, GRID, INIT_ANALYTIC, INIT_SEEDED, STAR, PrkError, check, lib, lib_path,
+    BF16, F32, F64, GRID, INIT_ANALYTIC, INIT_SEEDED, STAR, PrkError, Tuning, check, lib, lib_path,
 )
 from .ops import (  # noqa: F401
     DTYPES, add, alloc_grid, init, l1norm, nominal_flops, run, stencil, stencil_bytes_per_iter,

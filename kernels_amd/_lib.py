@@ -39,6 +39,24 @@ SIGNATURES = {
 }
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("variant", c_int), ("rows_per_wave", c_int), ("wgs_per_cu", c_int), ("order", c_int),
+                ("gbs", c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+SIGNATURES.update({
+    "prk_autotune": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+    "prk_tuning_get": (c_int, [c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+    "prk_tuning_set": (c_int, [c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+    "prk_tuning_clear": (c_int, []),
+    "prk_tuning_save": (c_int, [ctypes.c_char_p]),
+    "prk_tuning_load": (c_int, [ctypes.c_char_p]),
+})
+
+
 class PrkError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"prk error {code}: {msg}")

@@ -40,40 +40,15 @@ int check_hip(hipError_t e, const char *what)
     return set_error(PRK_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-static int g_rows_per_wave = 0;  // 0: automatic
+// Process-wide overrides of the tuned tiling (tests and tuning probes).
+static int g_rows_per_wave = 0;  // 0: tuned / default
 static int g_variant_override = -1;
-static int g_order = 0;
-static int g_wgs_per_cu = 0;
-
-// Measured on MI355X at n=30000 (FP64 star/grid R=2, FP32/BF16 star R=2):
-// FP64 prefers a 2-row pipeline with temporal stores, the narrower storage
-// types a 2-row pipeline with non-temporal out/in_next traffic.
-int default_variant(int dtype, int pattern, int radius)
-{
-    (void)pattern;
-    (void)radius;
-    return dtype == F64 ? 1 : 4;
-}
-
-int default_rows_per_wave() { return g_rows_per_wave; }
+static int g_order = -1;
+static int g_wgs_per_cu = -1;
 
 static bool valid_dtype(int dt) { return dt == F64 || dt == F32 || dt == BF16; }
 
 static int vec_width(int dt) { return dt == F64 ? 2 : dt == F32 ? 4 : 8; }
-
-// Row block per wave: long enough that the 2R halo rows each block re-reads
-// are a few percent, short enough that a grid yields several workgroups per
-// CU (256 CUs) so the tail of the launch stays short.
-static int auto_rows_per_wave(int dtype, int n)
-{
-    const int sw = 64 * vec_width(dtype);
-    const long long groups = ((n + sw - 1) / sw + kWavesPerBlock - 1) / kWavesPerBlock;
-    const long long target_blocks = 256LL * 16;
-    long long rows = (long long)n * groups / target_blocks;
-    rows = std::max<long long>(16, std::min<long long>(256, rows));
-    rows = (rows + 7) / 8 * 8;
-    return (int)rows;
-}
 
 int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
                const void *in_next, const void *out, int rows_per_wave, SweepPlan *plan)
@@ -104,11 +79,11 @@ int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const
     plan->pitch = pitch;
     plan->pad_writable = false;
     plan->mode = !aligned ? MODE_ELEM : (n % V == 0 ? MODE_VEC : MODE_VEC_LD);
-    plan->variant = g_variant_override >= 0 ? g_variant_override : default_variant(dtype, pattern, radius);
-    plan->order = g_order;
-    plan->wgs_per_cu = g_wgs_per_cu;
-    int rows = rows_per_wave > 0 ? rows_per_wave
-                                 : (g_rows_per_wave > 0 ? g_rows_per_wave : auto_rows_per_wave(dtype, n));
+    const prk_tuning t = choose_tuning(pattern, radius, dtype, n);
+    plan->variant = g_variant_override >= 0 ? g_variant_override : t.variant;
+    plan->order = g_order >= 0 ? g_order : t.order;
+    plan->wgs_per_cu = g_wgs_per_cu >= 0 ? g_wgs_per_cu : t.wgs_per_cu;
+    int rows = rows_per_wave > 0 ? rows_per_wave : (g_rows_per_wave > 0 ? g_rows_per_wave : t.rows_per_wave);
     plan->rows_per_wave = std::max(1, std::min(rows, n));
     return PRK_OK;
 }
@@ -378,8 +353,8 @@ int prk_set_sweep_variant(int variant)
 
 int prk_set_sweep_order(int order, int wgs_per_cu)
 {
-    if (order < 0 || order > 2 || wgs_per_cu < 0 || wgs_per_cu > 16)
-        return set_error(PRK_EINVAL, "order must be 0..2 and wgs_per_cu 0..16");
+    if (order < -1 || order > 3 || wgs_per_cu < -1 || wgs_per_cu > 16)
+        return set_error(PRK_EINVAL, "order must be -1..3 and wgs_per_cu -1..16 (-1: tuned)");
     g_order = order;
     g_wgs_per_cu = wgs_per_cu;
     return PRK_OK;

@@ -29,8 +29,11 @@ struct SweepPlan {
 
 // Number of fused-sweep variants (pipeline depth 1..3 x temporal/non-temporal).
 constexpr int kSweepVariants = 6;
-// Default variant for a (dtype, pattern, radius) before any tuning.
-int default_variant(int dtype, int pattern, int radius);
+
+// Tiling a sweep of this shape uses: tuned entry, nearest tuned size, or the
+// built-in default (autotune.hip).
+prk_tuning choose_tuning(int pattern, int radius, int dtype, int n);
+prk_tuning heuristic_tuning(int pattern, int radius, int dtype, int n);
 
 // Fills the launch geometry; rows_per_wave <= 0 picks automatically.
 int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
@@ -42,6 +45,5 @@ int launch_sweep_f64(const SweepPlan &p, const void *in, void *in_next, void *ou
 int launch_sweep_f32(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
 int launch_sweep_bf16(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s);
 
-int default_rows_per_wave();
 
 }  // namespace prk

@@ -68,7 +68,10 @@ struct SweepArgs {
 //   order 2: each XCD owns a contiguous range of column groups and walks it
 //            row-major, so the workgroups running at once on an XCD cover a
 //            narrow band of rows and a tile's vertical neighbour (whose rows
-//            it re-reads as halo) runs on the same XCD shortly after it.
+//            it re-reads as halo) runs on the same XCD shortly after it;
+//   order 3: row-major with the column-group count padded to a multiple of 8
+//            (chunk), so column group g always runs on XCD g%8: a tile and
+//            the one below it share an L2 and run one row band apart.
 __device__ inline bool tile_of(const SweepArgs &a, int b, int &group, int &rb)
 {
     if (a.order == 1) {
@@ -76,6 +79,11 @@ __device__ inline bool tile_of(const SweepArgs &a, int b, int &group, int &rb)
         rb = b / a.ngroups;
         group = b - rb * a.ngroups;
         return true;
+    }
+    if (a.order == 3) {
+        rb = b / a.chunk;
+        group = b - rb * a.chunk;
+        return rb < a.nrowblocks && group < a.ngroups;
     }
     const int x = b & 7, k = b >> 3;
     if (a.order == 2) {

@@ -91,3 +91,27 @@ def test_cli_argument_errors():
         r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == 1, (args, r.stdout)
         assert msg in r.stdout, (args, r.stdout)
+
+
+def test_tuning_table_roundtrip(tmp_path):
+    L = kernels_amd.lib()
+    T = kernels_amd.Tuning
+    check(L.prk_tuning_clear())
+    d = T()
+    check(L.prk_tuning_get(0, 2, 0, 30000, d))       # built-in default
+    assert 0 <= d.variant < 6 and d.rows_per_wave >= 1 and d.order == 1
+    t = T(variant=4, rows_per_wave=12, wgs_per_cu=3, order=1, gbs=5000.0)
+    check(L.prk_tuning_set(0, 2, 0, 30000, t))
+    g = T()
+    check(L.prk_tuning_get(0, 2, 0, 25000, g))       # nearest tuned size within 1.5x
+    assert (g.variant, g.rows_per_wave, g.wgs_per_cu) == (4, 12, 3)
+    check(L.prk_tuning_get(0, 2, 0, 10000, g))       # too far: default
+    assert (g.variant, g.rows_per_wave) == (d.variant, min(d.rows_per_wave, 10000))
+    path = str(tmp_path / "tune.txt").encode()
+    check(L.prk_tuning_save(path))
+    check(L.prk_tuning_clear())
+    assert L.prk_tuning_load(path) == 1
+    check(L.prk_tuning_get(0, 2, 0, 30000, g))
+    assert (g.variant, g.rows_per_wave, g.wgs_per_cu, g.gbs) == (4, 12, 3, 5000.0)
+    assert L.prk_tuning_set(0, 2, 0, 100, T(variant=9, rows_per_wave=8)) == -1
+    check(L.prk_tuning_clear())

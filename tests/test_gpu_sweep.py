@@ -171,3 +171,23 @@ def test_triad_and_copy(gpu):
     K.check(K.lib().prk_copy(n * 8, D.data_ptr(), A.data_ptr(), None))
     torch.cuda.synchronize()
     assert torch.equal(D[:n], A[:n])
+
+
+def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
+    n, iters, pattern, radius = 300, 3, 1, 3
+    L = K.lib()
+    best = K.Tuning()
+    K.check(L.prk_autotune(pattern, radius, K.F64, n, 1, best))
+    assert best.gbs > 0 and 1 <= best.rows_per_wave <= 32
+    ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 9)
+    try:
+        for v, rows, order, wgs in [(-1, 0, -1, -1), (0, 1, 0, 0), (3, 5, 2, 1), (5, 64, 1, 4), (2, 300, 1, 2)]:
+            K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
+            K.check(L.prk_set_sweep_order(order, wgs))
+            b0, b1, out = (K.alloc_grid(n, K.F64) for _ in range(3))
+            K.init(K.INIT_SEEDED, 9, K.F64, b0, out)
+            K.run(pattern, radius, K.F64, iters, b0, b1, out)
+            assert np.array_equal(bits(to_host(out)), bits(ref)), (v, rows, order, wgs)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
+        K.check(L.prk_set_sweep_order(-1, -1)); K.check(L.prk_tuning_clear())
