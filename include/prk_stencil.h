@@ -171,6 +171,54 @@ int prk_tuning_save(const char *path);
 int prk_tuning_load(const char *path);
 
 /* ------------------------------------------------------------------------
+ * Batched solves (no reference counterpart: the reference runs one grid per
+ * process).  A solve is the whole driver of Cxx11/stencil.cc:170-216 for one
+ * grid: initial values, iterations+1 fused sweeps, L1 norm.  A batch holds
+ * many solves of any sizes, patterns, radii (1..4), iteration counts and
+ * storage types; prk_batch_create allocates all their grids in one device
+ * pool (pitch rounded up to 16 bytes; padding columns are scratch), cuts a
+ * sliding-window schedule into rounds (one stage of each active solve per
+ * round, one launch per round over the tiles of all its solves) and
+ * captures the whole run in a hipGraph, one branch per storage type.  Every
+ * solve's arithmetic is that of the single-grid sweep, so each result is
+ * the same whatever else is in the batch.  rows_per_tile: tile height
+ * (0 = default by radius).  flags: 0, or PRK_BATCH_PERSISTENT for the
+ * single-launch tile-queue form, which is disabled (it hung on MI355X) and
+ * returns PRK_EINVAL.
+ *   prk_batch_run     enqueue one run of every solve on `stream`
+ *   prk_batch_wait    wait for it; norms[count] receives the L1 norms
+ *   prk_batch_output  copy solve idx's out (which=0) or final in (which=1)
+ *                     grid to host, dense n*n
+ *   prk_batch_info    device bytes, total tiles, launches per run
+ * prk_solve_one_by_one runs the same solves one grid per launch through the
+ * single-grid path (init, fused sweeps, norm), optionally forcing the
+ * storage type (dtype_override >= 0): the unbatched baseline.
+ * ------------------------------------------------------------------------ */
+typedef struct prk_solve {
+    int pattern;
+    int radius;
+    int dtype;
+    int n;
+    int iterations;
+    int init;
+    uint64_t seed;
+} prk_solve;
+
+typedef struct prk_batch prk_batch;
+
+#define PRK_BATCH_PERSISTENT 1
+
+int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int flags,
+                     prk_batch **out);
+int prk_batch_run(prk_batch *b, void *stream);
+int prk_batch_wait(prk_batch *b, double *norms);
+int prk_batch_output(const prk_batch *b, int idx, int which, void *host, size_t bytes);
+int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles, int *launches);
+void prk_batch_destroy(prk_batch *b);
+int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override, double *norms,
+                         void *stream);
+
+/* ------------------------------------------------------------------------
  * HBM calibration (nstream triad A += B + s*C, Cxx11/nstream-hip.cc:67-80;
  * bytes 4*len*8 per call, :221) and a plain copy dst = src.
  * ------------------------------------------------------------------------ */

@@ -57,6 +57,23 @@ SIGNATURES.update({
 })
 
 
+class Solve(ctypes.Structure):
+    _fields_ = [("pattern", c_int), ("radius", c_int), ("dtype", c_int), ("n", c_int), ("iterations", c_int),
+                ("init", c_int), ("seed", c_u64)]
+
+
+_SOLVE_P = ctypes.POINTER(Solve)
+SIGNATURES.update({
+    "prk_batch_create": (c_int, [_SOLVE_P, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "prk_batch_run": (c_int, [c_void_p, c_void_p]),
+    "prk_batch_wait": (c_int, [c_void_p, c_double_p]),
+    "prk_batch_output": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_size_t]),
+    "prk_batch_info": (c_int, [c_void_p, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll), c_int_p]),
+    "prk_batch_destroy": (None, [c_void_p]),
+    "prk_solve_one_by_one": (c_int, [_SOLVE_P, c_int, c_int, c_double_p, c_void_p]),
+})
+
+
 class PrkError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"prk error {code}: {msg}")

@@ -1,0 +1,364 @@
+// Host side of the batched solve path: one device pool for every solve's
+// grids, a sliding-window schedule cut into rounds (a round holds at most one
+// stage of each solve), one launch per round, all rounds of a storage type on
+// one stream and the storage types on parallel branches of one captured
+// hipGraph.  Also the one-grid-per-launch baseline that runs the same solves
+// through the single-grid sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "batch.hpp"
+#include "prk_internal.hpp"
+
+namespace prk {
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int vec_of(int dt) { return dt == F64 ? 2 : dt == F32 ? 4 : 8; }
+
+// Rows per tile: short blocks keep the waves of a launch on a compact band
+// of rows; the 2R halo rows each block re-reads grow with the radius.
+int default_tile_rows(int radius) { return radius <= 2 ? 8 : radius == 3 ? 12 : 16; }
+
+// Tiles a round aims for: several waves per CU on all 256 CUs, so a round's
+// tail (its slowest waves) is a small part of it.
+constexpr long long kRoundTiles = 16384;
+
+struct Round {
+    size_t first_item;
+    int nitems;
+    int waves;
+};
+
+}  // namespace
+
+}  // namespace prk
+
+using namespace prk;
+
+struct prk_batch {
+    std::vector<prk_solve> solves;
+    std::vector<BGrid> grids;  // host copies
+    void *pool = nullptr;      // grids
+    size_t pool_bytes = 0;
+    void *meta = nullptr;      // descriptors, items, partials, norms
+    BGrid *d_grids = nullptr;
+    RItem *d_items = nullptr;
+    double *d_norms = nullptr;
+    std::vector<Round> rounds[3];
+    long long total_tiles = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t capture = nullptr;
+    hipStream_t branch[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t done = nullptr;
+    bool launched = false;
+};
+
+namespace {
+
+int validate_solve(const prk_solve &s)
+{
+    if (s.pattern != PRK_STAR && s.pattern != PRK_GRID) return set_error(PRK_EINVAL, "bad pattern %d", s.pattern);
+    if (s.radius < 1 || s.radius > kBatchMaxRadius)
+        return set_error(PRK_ENOSTENCIL, "batched solves support radius 1..%d, got %d", kBatchMaxRadius, s.radius);
+    if (s.dtype != PRK_F64 && s.dtype != PRK_F32 && s.dtype != PRK_BF16)
+        return set_error(PRK_EINVAL, "bad dtype %d", s.dtype);
+    if (s.n < 2 * s.radius + 1) return set_error(PRK_EINVAL, "n=%d too small for radius %d", s.n, s.radius);
+    if (s.n > 92680) return set_error(PRK_EINVAL, "n=%d too large", s.n);
+    if (s.iterations < 0) return set_error(PRK_EINVAL, "iterations must be >= 0");
+    if (s.init != PRK_INIT_ANALYTIC && s.init != PRK_INIT_SEEDED) return set_error(PRK_EINVAL, "bad init");
+    return PRK_OK;
+}
+
+int stage_tiles(const BGrid &g, int stage) { return stage == g.iters + 3 ? 1 : g.ntiles; }
+
+// Sliding window: keep enough solves active that a round of their next
+// stages holds about `target` tiles; each round takes one stage of every
+// active solve; a solve leaves after its final-norm stage and the next one
+// in submission order takes its place.
+void schedule_rounds(const std::vector<int> &ids, const std::vector<BGrid> &grids, long long target,
+                     std::vector<RItem> &items, std::vector<Round> &rounds)
+{
+    std::vector<int> active;
+    std::vector<int> stage(grids.size(), 0);
+    size_t next = 0;
+    long long in_flight = 0;
+    while (!active.empty() || next < ids.size()) {
+        while (next < ids.size() && (active.empty() || in_flight < target)) {
+            active.push_back(ids[next]);
+            in_flight += grids[ids[next]].ntiles;
+            ++next;
+        }
+        Round r{items.size(), 0, 0};
+        std::vector<int> keep;
+        for (int g : active) {
+            items.push_back(RItem{g, stage[g], r.waves});
+            r.waves += stage_tiles(grids[g], stage[g]);
+            ++r.nitems;
+            if (++stage[g] <= grids[g].iters + 3) keep.push_back(g);
+            else in_flight -= grids[g].ntiles;
+        }
+        rounds.push_back(r);
+        active.swap(keep);
+    }
+}
+
+void batch_free(prk_batch *b)
+{
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (int d = 0; d < 3; ++d) {
+        if (b->branch[d]) (void)hipStreamDestroy(b->branch[d]);
+        if (b->join[d]) (void)hipEventDestroy(b->join[d]);
+    }
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->pool) (void)hipFree(b->pool);
+    if (b->meta) (void)hipFree(b->meta);
+    delete b;
+}
+
+int launch_round(int dt, const RoundArgs &a, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((a.total + kWavesPerBlock - 1) / kWavesPerBlock);
+    switch (dt) {
+    case F64: hipLaunchKernelGGL(batch_round_kernel<F64>, dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case F32: hipLaunchKernelGGL(batch_round_kernel<F32>, dim3(blocks), dim3(kThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL(batch_round_kernel<BF16>, dim3(blocks), dim3(kThreads), 0, s, a); break;
+    }
+    return check_hip(hipGetLastError(), "batch round launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int flags, prk_batch **out)
+{
+    if (!solves || count < 1 || !out) return set_error(PRK_EINVAL, "need at least one solve");
+    if (rows_per_tile < 0) return set_error(PRK_EINVAL, "rows_per_tile must be >= 0");
+    if (flags & PRK_BATCH_PERSISTENT)
+        return set_error(PRK_EINVAL, "the persistent tile-queue form is disabled: it hung on MI355X");
+    if (flags & ~PRK_BATCH_PERSISTENT) return set_error(PRK_EINVAL, "unknown batch flags 0x%x", flags);
+    int rc;
+    for (int i = 0; i < count; ++i)
+        if ((rc = validate_solve(solves[i]))) return rc;
+
+    prk_batch *b = new prk_batch();
+    b->solves.assign(solves, solves + count);
+    b->grids.resize(count);
+
+    // grid pool layout
+    size_t off = 0;
+    std::vector<size_t> offs(count);
+    size_t partial_total = 0;
+    for (int i = 0; i < count; ++i) {
+        const prk_solve &s = solves[i];
+        const int V = vec_of(s.dtype);
+        BGrid &g = b->grids[i];
+        std::memset(&g, 0, sizeof g);
+        g.pitch = (long long)align_up((size_t)s.n, (size_t)V);
+        g.n = s.n;
+        g.pattern = s.pattern;
+        g.radius = s.radius;
+        g.iters = s.iterations;
+        g.rows = std::min(s.n, rows_per_tile > 0 ? rows_per_tile : default_tile_rows(s.radius));
+        g.strips = (s.n + 64 * V - 1) / (64 * V);
+        g.ntiles = g.strips * ((s.n + g.rows - 1) / g.rows);
+        g.init_kind = s.init;
+        g.seed = s.seed;
+        offs[i] = off;
+        off += 3 * align_up((size_t)s.n * (size_t)g.pitch * dtype_size(s.dtype), 256);
+        partial_total += (size_t)g.ntiles;
+    }
+    b->pool_bytes = off;
+    rc = check_hip(hipMalloc(&b->pool, off), "batch grid pool");
+    if (rc) {
+        batch_free(b);
+        return set_error(PRK_ENOMEM, "batch needs %zu bytes of device memory", off);
+    }
+    for (int i = 0; i < count; ++i) {
+        BGrid &g = b->grids[i];
+        const size_t arr = align_up((size_t)g.n * (size_t)g.pitch * dtype_size(solves[i].dtype), 256);
+        g.buf0 = (char *)b->pool + offs[i];
+        g.buf1 = (char *)g.buf0 + arr;
+        g.out = (char *)g.buf1 + arr;
+    }
+
+    // one schedule per storage type
+    std::vector<RItem> items;
+    for (int dt = 0; dt < 3; ++dt) {
+        std::vector<int> ids;
+        for (int i = 0; i < count; ++i)
+            if (solves[i].dtype == dt) ids.push_back(i);
+        if (!ids.empty()) schedule_rounds(ids, b->grids, kRoundTiles, items, b->rounds[dt]);
+    }
+    for (int dt = 0; dt < 3; ++dt)
+        for (const Round &r : b->rounds[dt]) b->total_tiles += r.waves;
+
+    // metadata block
+    size_t m = 0;
+    const size_t o_grids = m;
+    m = align_up(m + sizeof(BGrid) * count, 256);
+    const size_t o_items = m;
+    m = align_up(m + sizeof(RItem) * items.size(), 256);
+    const size_t o_norms = m;
+    m = align_up(m + sizeof(double) * count, 256);
+    const size_t o_part = m;
+    m = align_up(m + sizeof(DDv) * partial_total, 256);
+    rc = check_hip(hipMalloc(&b->meta, m), "batch metadata");
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    char *mb = (char *)b->meta;
+    b->d_grids = (BGrid *)(mb + o_grids);
+    b->d_items = (RItem *)(mb + o_items);
+    b->d_norms = (double *)(mb + o_norms);
+    DDv *parts = (DDv *)(mb + o_part);
+    size_t pk = 0;
+    for (int i = 0; i < count; ++i) {
+        b->grids[i].partials = parts + pk;
+        pk += (size_t)b->grids[i].ntiles;
+        b->grids[i].norm = b->d_norms + i;
+    }
+    rc = check_hip(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * count, hipMemcpyHostToDevice), "grids");
+    if (!rc)
+        rc = check_hip(hipMemcpy(b->d_items, items.data(), sizeof(RItem) * items.size(), hipMemcpyHostToDevice),
+                       "items");
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+
+    // capture: fork one branch per storage type, its rounds in order, join
+    rc = check_hip(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "batch stream");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "batch event");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event");
+    for (int dt = 0; dt < 3 && !rc; ++dt) {
+        if (b->rounds[dt].empty()) continue;
+        rc = check_hip(hipStreamCreateWithFlags(&b->branch[dt], hipStreamNonBlocking), "batch branch stream");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->join[dt], hipEventDisableTiming), "batch event");
+    }
+    if (!rc) rc = check_hip(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (!rc) {
+        rc = check_hip(hipEventRecord(b->fork, b->capture), "fork");
+        for (int dt = 0; dt < 3 && !rc; ++dt) {
+            if (b->rounds[dt].empty()) continue;
+            rc = check_hip(hipStreamWaitEvent(b->branch[dt], b->fork, 0), "fork wait");
+            for (const Round &r : b->rounds[dt]) {
+                if (rc) break;
+                RoundArgs a{b->d_grids, b->d_items + r.first_item, r.nitems, r.waves};
+                rc = launch_round(dt, a, b->branch[dt]);
+            }
+            if (!rc) rc = check_hip(hipEventRecord(b->join[dt], b->branch[dt]), "join");
+            if (!rc) rc = check_hip(hipStreamWaitEvent(b->capture, b->join[dt], 0), "join wait");
+        }
+        hipGraph_t g = nullptr;
+        const int rc2 = check_hip(hipStreamEndCapture(b->capture, &g), "end capture");
+        b->graph = g;
+        if (!rc) rc = rc2;
+    }
+    if (!rc) rc = check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return PRK_OK;
+}
+
+int prk_batch_run(prk_batch *b, void *stream)
+{
+    if (!b) return set_error(PRK_EINVAL, "null batch");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = check_hip(hipGraphLaunch(b->exec, s), "batch graph launch");
+    if (!rc) rc = check_hip(hipEventRecord(b->done, s), "batch event");
+    if (!rc) b->launched = true;
+    return rc;
+}
+
+int prk_batch_wait(prk_batch *b, double *norms)
+{
+    if (!b || !b->launched) return set_error(PRK_EINVAL, "batch has not been run");
+    int rc = check_hip(hipEventSynchronize(b->done), "batch wait");
+    if (!rc && norms)
+        rc = check_hip(hipMemcpy(norms, b->d_norms, sizeof(double) * b->solves.size(), hipMemcpyDeviceToHost),
+                       "batch norms");
+    return rc;
+}
+
+int prk_batch_output(const prk_batch *b, int idx, int which, void *host, size_t bytes)
+{
+    if (!b || idx < 0 || idx >= (int)b->solves.size() || !host || which < 0 || which > 1)
+        return set_error(PRK_EINVAL, "bad batch output request");
+    const BGrid &g = b->grids[idx];
+    const size_t es = dtype_size(b->solves[idx].dtype);
+    if (bytes < (size_t)g.n * g.n * es) return set_error(PRK_EINVAL, "host buffer too small");
+    const void *src = which == 0 ? g.out : ((b->solves[idx].iterations + 1) % 2 ? g.buf1 : g.buf0);
+    return check_hip(hipMemcpy2D(host, (size_t)g.n * es, src, (size_t)g.pitch * es, (size_t)g.n * es, (size_t)g.n,
+                                 hipMemcpyDeviceToHost),
+                     "batch output copy");
+}
+
+int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles, int *launches)
+{
+    if (!b) return set_error(PRK_EINVAL, "null batch");
+    if (device_bytes) *device_bytes = (long long)b->pool_bytes;
+    if (tiles) *tiles = b->total_tiles;
+    if (launches) *launches = (int)(b->rounds[0].size() + b->rounds[1].size() + b->rounds[2].size());
+    return PRK_OK;
+}
+
+void prk_batch_destroy(prk_batch *b) { batch_free(b); }
+
+int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override, double *norms, void *stream)
+{
+    if (!solves || count < 1) return set_error(PRK_EINVAL, "need at least one solve");
+    int rc;
+    std::vector<prk_solve> ss(solves, solves + count);
+    size_t need = 0;
+    for (auto &s : ss) {
+        if (dtype_override >= 0) s.dtype = dtype_override;
+        if ((rc = validate_solve(s))) return rc;
+        const size_t V = (size_t)vec_of(s.dtype);
+        need = std::max(need, (size_t)s.n * align_up((size_t)s.n, V) * dtype_size(s.dtype));
+    }
+    need = align_up(need, 256);
+    void *mem = nullptr;
+    double *d_norms = nullptr;
+    if ((rc = check_hip(hipMalloc(&mem, 3 * need), "one-by-one buffers"))) return rc;
+    if ((rc = check_hip(hipMalloc(&d_norms, sizeof(double) * count), "one-by-one norms"))) {
+        (void)hipFree(mem);
+        return rc;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    void *b0 = mem, *b1 = (char *)mem + need, *out = (char *)mem + 2 * need;
+    for (int i = 0; i < count && rc == PRK_OK; ++i) {
+        const prk_solve &s = ss[i];
+        const long long pitch = (long long)align_up((size_t)s.n, (size_t)vec_of(s.dtype));
+        rc = prk_init(s.init, s.seed, s.dtype, s.n, pitch, b0, out, st);
+        SweepPlan p;
+        if (!rc) rc = plan_sweep(s.pattern, s.radius, s.dtype, s.n, pitch, b0, b1, out, 0, &p);
+        void *bufs[2] = {b0, b1};
+        for (int k = 0; k <= s.iterations && rc == PRK_OK; ++k)
+            rc = launch_sweep(p, bufs[k & 1], bufs[(k + 1) & 1], out, st);
+        if (!rc) rc = l1norm_async(s.dtype, s.n, pitch, s.radius, out, d_norms + i, st);
+    }
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "one-by-one sync");
+    if (!rc && norms) rc = check_hip(hipMemcpy(norms, d_norms, sizeof(double) * count, hipMemcpyDeviceToHost), "norms");
+    (void)hipFree(d_norms);
+    (void)hipFree(mem);
+    return rc;
+}
+
+}  // extern "C"

@@ -324,6 +324,17 @@ __global__ __launch_bounds__(256) void copy_kernel(long long nvec, u32x4 *dst, c
     if (k < nvec) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[k]), &dst[k]);
 }
 
+int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, hipStream_t s)
+{
+    if (!valid_dtype(dtype)) return set_error(PRK_EINVAL, "unknown dtype %d", dtype);
+    if (n < 1 || pitch < n || !out || !d_result || radius < 1 || 2 * radius + 1 > n)
+        return set_error(PRK_EINVAL, "bad norm arguments (n=%d radius=%d)", n, radius);
+    NormWorkspace *ws = nullptr;
+    int rc = norm_workspace(&ws);
+    if (rc) return rc;
+    return enqueue_l1norm(dtype, n, pitch, radius, out, d_result, ws->partials, s);
+}
+
 }  // namespace prk
 
 using namespace prk;

@@ -30,6 +30,10 @@ struct SweepPlan {
 // Number of fused-sweep variants (pipeline depth 1..3 x temporal/non-temporal).
 constexpr int kSweepVariants = 6;
 
+// L1 norm into a device double, enqueued on s (shares one per-device
+// workspace: norms on different streams must not overlap).
+int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, hipStream_t s);
+
 // Tiling a sweep of this shape uses: tuned entry, nearest tuned size, or the
 // built-in default (autotune.hip).
 prk_tuning choose_tuning(int pattern, int radius, int dtype, int n);

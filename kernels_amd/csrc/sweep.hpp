@@ -281,42 +281,34 @@ __device__ inline void bstore_elem(__amdgpu_buffer_rsrc_t r, unsigned off, T v)
 //   MODE_ELEM   element loads and stores (rows not 16-byte aligned).
 enum { MODE_ELEM = 0, MODE_VEC = 1, MODE_VEC_LD = 2 };
 
-template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
-__global__ __launch_bounds__(kThreads)
-#if PRK_SWEEP_WAVES_PER_EU > 0
-__attribute__((amdgpu_waves_per_eu(PRK_SWEEP_WAVES_PER_EU)))
-#endif
-void sweep_kernel(SweepArgs a)
+// One wavefront's tile: columns [c0, c0+64V) of rows [r0, r1) of one grid.
+// ring points at this wave's (2R+1) x LR element LDS ring.  LDAUX adds cache
+// policy bits to every HBM load (the batch kernel bypasses L1 with it, since
+// neighbouring tiles written by other CUs are read there).
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX>
+__device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n, int c0,
+                                 int r0, int r1, int pad_writable, typename DTypeTraits<DT>::T *ring_base,
+                                 int lane)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
     constexpr int V = G::V, SW = G::SW, HP = G::HP, NH = G::NH, LR = G::LR, S = G::S;
     constexpr int ES = (int)sizeof(T);
     constexpr int AUX_STREAM = (NT & 1) ? 2 : 0;  // nt bit on out loads and all stores
-    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][S][LR];
+    constexpr int AUX_IN = LDAUX;
+    constexpr int AUX_OUT_LD = AUX_STREAM | LDAUX;
+    (void)SW;
 
-    int group, rb;
-    if (!tile_of(a, blockIdx.x, group, rb)) return;
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int strip = group * kWavesPerBlock + wave;
-    if (strip >= a.nstrips) return;
-
-    const int n = a.n;
-    const long long pitch = a.pitch;
-    const int c0 = strip * SW;
-    const int r0 = rb * a.rows_per_wave;
-    const int r1 = min(r0 + a.rows_per_wave, n);
     const int lo = max(r0 - R, 0), hi = min(r1 + R, n);
     const int olo = max(r0, R), ohi = min(r1, n - R);
 
-    const T *in = (const T *)a.in;
-    T *in_next = (T *)a.in_next;
-    T *out = (T *)a.out;
+    const T *in = (const T *)in_;
+    T *in_next = (T *)in_next_;
+    T *out = (T *)out_;
     // Loads may read a whole padded row; stores stay inside [0, n) unless the
     // caller allows the padding to be written.
     const unsigned ld_bytes = (unsigned)(pitch * ES);
-    const unsigned st_bytes = a.pad_writable ? ld_bytes : (unsigned)(n * ES);
+    const unsigned st_bytes = pad_writable ? ld_bytes : (unsigned)(n * ES);
 
     const int cm = c0 + lane * V;  // first column of this lane's main vector
     // Halo vector of this lane (lanes 0..NH-1 left, NH..2NH-1 right); other
@@ -330,7 +322,7 @@ void sweep_kernel(SweepArgs a)
 #pragma unroll
     for (int v = 0; v < V; ++v) interior[v] = (cm + v >= R) && (cm + v < n - R);
 
-    T(*ring)[LR] = lds[wave];
+    T(*ring)[LR] = (T(*)[LR])ring_base;
 
     // Row pipeline: buf[k] holds input row (main + halo) and the out row R
     // above it, loaded PF steps ahead.
@@ -343,15 +335,15 @@ void sweep_kernel(SweepArgs a)
         const bool out_ok = o >= olo && o < ohi;
         const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, ld_bytes);
         if constexpr (MODE != MODE_ELEM) {
-            r.main.v = bload16<0>(rin, row_ok ? off_main : kOOB);
-            r.halo.v = bload16<0>(rin, row_ok ? off_halo : kOOB);
-            r.outv.v = bload16<AUX_STREAM>(rout, out_ok ? off_main : kOOB);
+            r.main.v = bload16<AUX_IN>(rin, row_ok ? off_main : kOOB);
+            r.halo.v = bload16<AUX_IN>(rin, row_ok ? off_halo : kOOB);
+            r.outv.v = bload16<AUX_OUT_LD>(rout, out_ok ? off_main : kOOB);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                r.main.e[v] = bload_elem<T, 0>(rin, row_ok ? off_main + v * ES : kOOB);
-                r.halo.e[v] = bload_elem<T, 0>(rin, (row_ok && off_halo != kOOB) ? off_halo + v * ES : kOOB);
-                r.outv.e[v] = bload_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB);
+                r.main.e[v] = bload_elem<T, AUX_IN>(rin, row_ok ? off_main + v * ES : kOOB);
+                r.halo.e[v] = bload_elem<T, AUX_IN>(rin, (row_ok && off_halo != kOOB) ? off_halo + v * ES : kOOB);
+                r.outv.e[v] = bload_elem<T, AUX_OUT_LD>(rout, out_ok ? off_main + v * ES : kOOB);
             }
         }
     };
@@ -419,6 +411,30 @@ void sweep_kernel(SweepArgs a)
             slot = (slot + 1 == S) ? 0 : slot + 1;
         }
     }
+}
+
+
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+__global__ __launch_bounds__(kThreads)
+#if PRK_SWEEP_WAVES_PER_EU > 0
+__attribute__((amdgpu_waves_per_eu(PRK_SWEEP_WAVES_PER_EU)))
+#endif
+void sweep_kernel(SweepArgs a)
+{
+    typedef SweepGeom<DT, R> G;
+    typedef typename G::T T;
+    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][G::S][G::LR];
+
+    int group, rb;
+    if (!tile_of(a, blockIdx.x, group, rb)) return;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int strip = group * kWavesPerBlock + wave;
+    if (strip >= a.nstrips) return;
+    const int r0 = rb * a.rows_per_wave;
+    const int r1 = min(r0 + a.rows_per_wave, a.n);
+    wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, strip * G::SW, r0, r1,
+                                               a.pad_writable, &lds[wave][0][0], lane);
 }
 
 }  // namespace prk

@@ -115,3 +115,16 @@ def test_tuning_table_roundtrip(tmp_path):
     assert (g.variant, g.rows_per_wave, g.wgs_per_cu, g.gbs) == (4, 12, 3, 5000.0)
     assert L.prk_tuning_set(0, 2, 0, 100, T(variant=9, rows_per_wave=8)) == -1
     check(L.prk_tuning_clear())
+
+
+def test_batch_persistent_form_is_refused():
+    import ctypes
+    from kernels_amd._lib import Solve
+    arr = (Solve * 1)(Solve(0, 2, 1, 64, 1, 1, 0))
+    h = ctypes.c_void_p()
+    L = kernels_amd.lib()
+    assert L.prk_batch_create(arr, 1, 0, 1, ctypes.byref(h)) == -1
+    assert b"disabled" in L.prk_last_error()
+    assert L.prk_batch_create(arr, 1, 0, 4, ctypes.byref(h)) == -1
+    bad = (Solve * 1)(Solve(0, 5, 1, 64, 1, 1, 0))
+    assert L.prk_batch_create(bad, 1, 0, 0, ctypes.byref(h)) == -2   # radius 5 not batched

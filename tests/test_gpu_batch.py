@@ -1,0 +1,114 @@
+"""Batched solves against the oracle, alone vs in a mix, and across graph
+replays.  Every test runs in the round form (one launch per round of the
+schedule).  The persistent tile-queue form hung on MI355X (a wave waiting on
+another wave's completion counter never saw it advance); it is disabled in
+the library and its cases stay here, skipped."""
+import numpy as np
+import pytest
+
+from kernels_amd.batch import Batch, one_by_one
+from tests._oracle import BF16, F32, F64
+
+pytestmark = pytest.mark.gpu
+
+PERSISTENT_SKIP = pytest.mark.skip(reason="persistent tile-queue form hung on MI355X; disabled (PRK_BATCH_PERSISTENT)")
+FORMS = [pytest.param(0, id="rounds"), pytest.param(1, id="persistent", marks=PERSISTENT_SKIP)]
+
+
+def ref_solve(oracle, s):
+    pattern, radius, dtype, n, iters, init, seed = s
+    if dtype == F64:
+        out, fin = oracle.run(pattern, radius, n, 32, iters, init, seed)
+        norm = oracle.l1norm(n, radius, out)
+    else:
+        out, fin = oracle.run_lp(dtype, pattern, radius, n, iters, init, seed)
+        norm = oracle.l1norm_lp(dtype, n, radius, out)
+    return out, fin, norm
+
+
+MIX = [
+    # pattern, radius, dtype, n, iterations, init, seed
+    (0, 2, F64, 130, 3, 1, 11),
+    (1, 3, F32, 77, 5, 1, 12),
+    (1, 4, BF16, 300, 2, 1, 13),
+    (0, 1, F32, 1100, 1, 0, 0),
+    (1, 2, F64, 9, 0, 1, 14),
+    (0, 4, F64, 257, 4, 0, 0),
+    (1, 1, BF16, 64, 6, 0, 0),
+    (0, 3, F32, 515, 2, 1, 15),
+    (1, 4, F32, 33, 3, 1, 16),
+    (0, 2, BF16, 96, 1, 1, 17),
+]
+
+
+@pytest.mark.parametrize("flags", FORMS)
+def test_batch_matches_oracle(gpu, oracle, flags):
+    b = Batch(MIX, flags=flags)
+    b.run()
+    norms = b.wait()
+    for i, s in enumerate(MIX):
+        print(f"solve {i}: {s}", flush=True)
+        out, fin, norm = ref_solve(oracle, s)
+        got = b.output(i, 0)
+        assert np.array_equal(got.view(np.uint8), out.view(np.uint8)), f"out differs for {s}"
+        assert np.array_equal(b.output(i, 1).view(np.uint8), fin.view(np.uint8)), f"in differs for {s}"
+        assert abs(norms[i] - norm) <= 1e-13 * abs(norm) + 1e-300, (s, norms[i], norm)
+    b.close()
+
+
+@pytest.mark.parametrize("flags", FORMS)
+def test_alone_equals_in_mix_and_replays(gpu, flags):
+    mix = MIX * 3
+    b = Batch(mix, rows_per_tile=12, flags=flags)
+    runs = []
+    for _ in range(3):          # graph replays, with eager work in between
+        b.run()
+        runs.append(b.wait())
+        _ = one_by_one(MIX[:2])
+    for r in runs[1:]:
+        assert np.array_equal(r, runs[0])
+    outs = [b.output(i) for i in range(len(mix))]
+    for i, s in enumerate(MIX):
+        alone = Batch([s], flags=flags)
+        alone.run()
+        na = alone.wait()
+        assert na[0] == runs[0][i] == runs[0][i + len(MIX)]
+        assert np.array_equal(alone.output(0).view(np.uint8), outs[i].view(np.uint8))
+        alone.close()
+    b.close()
+
+
+def test_one_by_one_baseline(gpu, oracle):
+    specs = [s for s in MIX if s[2] == F64]
+    norms = one_by_one(specs)
+    for s, nm in zip(specs, norms):
+        ref = ref_solve(oracle, s)[2]
+        assert abs(nm - ref) <= 1e-13 * abs(ref)
+    # forced FP64 storage of the reduced-precision solves
+    n64 = one_by_one(MIX, dtype_override=F64)
+    for s, nm in zip(MIX, n64):
+        s64 = (s[0], s[1], F64) + s[3:]
+        ref = ref_solve(oracle, s64)[2]
+        assert abs(nm - ref) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("flags", FORMS)
+def test_batch_many_small(gpu, oracle, flags):
+    rng = np.random.default_rng(5)
+    specs = []
+    for i in range(300):
+        n = int(np.exp(rng.uniform(np.log(20), np.log(400))))
+        r = int(rng.integers(1, 5))
+        n = max(n, 2 * r + 1)
+        specs.append((int(rng.integers(0, 2)), r, int(rng.integers(0, 3)), n, int(rng.integers(0, 8)), 1, i))
+    b = Batch(specs, flags=flags)
+    b.run()
+    norms = b.wait()
+    for i in range(0, 300, 7):
+        _, _, ref = ref_solve(oracle, specs[i])
+        assert abs(norms[i] - ref) <= 1e-13 * abs(ref), specs[i]
+        out, _, _ = ref_solve(oracle, specs[i])
+        assert np.array_equal(b.output(i).view(np.uint8), out.view(np.uint8)), specs[i]
+    info = b.info()
+    assert info["launches"] >= 1 and info["tiles"] > 300
+    b.close()
