@@ -39,6 +39,98 @@ WORKLOADS = {
     "star2_bf16_n30000": (0, 2, 2, 30000),
 }
 DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
+BATCH_WORKLOADS = ("batch4096_f32",)
+
+
+def config4_solves(seed: int = 7, count: int = 4096):
+    """4096 uneven grids: n log-uniform in [64, 2048], star/grid 50/50, radius
+    uniform in 1..4, iterations uniform in 1..50, FP32 storage, seeded
+    U(-1,1) initial values (solve i uses seed i)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(64), np.log(2048), count)).round().astype(int)
+    pats = rng.integers(0, 2, count)
+    radii = rng.integers(1, 5, count)
+    iters = rng.integers(1, 51, count)
+    return [(int(p), int(r), 1, int(n), int(k), 1, i) for i, (p, r, n, k) in enumerate(zip(pats, radii, ns, iters))]
+
+
+def solve_bytes(spec) -> int:
+    """Algorithmic bytes of one solve: init writes in and out, each of the
+    K+1 fused sweeps reads in, writes in_next, reads and writes the interior
+    of out, the norm reads the interior once."""
+    pattern, radius, dtype, n, iters = spec[:5]
+    s = {0: 8, 1: 4, 2: 2}[dtype]
+    m = n - 2 * radius
+    return 2 * s * n * n + (iters + 1) * (2 * s * n * n + 2 * s * m * m) + s * m * m
+
+
+def bench_batch(a) -> int:
+    import numpy as np
+    import torch
+    import kernels_amd as K
+    from kernels_amd.batch import Batch, one_by_one
+
+    specs = config4_solves()
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    b = Batch(specs)
+    t_create = time.perf_counter() - t0
+    info = b.info()
+    for _ in range(a.warmup):
+        b.run(stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        b.run(stream.cuda_stream)
+    e1.record(stream)
+    norms = b.wait()
+    ms = e0.elapsed_time(e1)
+    b.close()
+    torch.cuda.empty_cache()
+    solves_s = len(specs) * a.steps / (ms * 1e-3)
+    total_bytes = sum(solve_bytes(s) for s in specs)
+    # unbatched baselines: the same solves one grid per launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n32 = one_by_one(specs)
+    t32 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    n64 = one_by_one(specs, dtype_override=K.F64)
+    t64 = time.perf_counter() - t0
+    rel32 = float(np.max(np.abs(norms - n32) / np.abs(n32)))
+    rel64 = float(np.max(np.abs(norms - n64) / np.abs(n64)))
+    res = {
+        "metric": METRIC,
+        "value": round(solves_s, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (4096 seeded U(-1,1) grids, generator seed 7)",
+        "config": {"workload": "batched uneven grids: 4096 solves, n log-uniform [64,2048], star/grid 50/50, "
+                               "R in 1..4, iterations 1..50, FP32 storage; one hipGraph of round launches",
+                   "solves": len(specs), "launches_per_run": info["launches"], "tiles_per_run": info["tiles"],
+                   "device_bytes": info["device_bytes"]},
+        "effective_gbs": round(total_bytes * a.steps / (ms * 1e-3) / 1e9, 1),
+        "algorithmic_bytes_per_run": total_bytes,
+        "one_by_one_f64": {"solves_per_s": round(len(specs) / t64, 1), "seconds": round(t64, 3),
+                           "max_rel_norm_diff_vs_batch": rel64},
+        "one_by_one_f32": {"solves_per_s": round(len(specs) / t32, 1), "seconds": round(t32, 3),
+                           "max_rel_norm_diff_vs_batch": rel32},
+        "speedup_vs_one_by_one_f64": round(solves_s * t64 / len(specs), 2),
+        "speedup_vs_one_by_one_f32": round(solves_s * t32 / len(specs), 2),
+        "create_s": round(t_create, 2),
+    }
+    print(json.dumps(res))
+    return 0 if rel32 < 1e-12 else 1
 
 
 def traffic_from_profiles(workload: str):
@@ -109,7 +201,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="star2_f64_n30000", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="star2_f64_n30000", choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-peak", action="store_true", help="skip the copy/triad peak measurement")
     ap.add_argument("--no-tune", action="store_true",
@@ -126,6 +218,9 @@ def main(argv=None) -> int:
     if a.steps < 1 or a.warmup < 0:
         print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
         return 2
+
+    if a.workload in BATCH_WORKLOADS:
+        return bench_batch(a)
 
     import torch
     import kernels_amd as K

@@ -215,6 +215,9 @@ int prk_batch_wait(prk_batch *b, double *norms);
 int prk_batch_output(const prk_batch *b, int idx, int which, void *host, size_t bytes);
 int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles, int *launches);
 void prk_batch_destroy(prk_batch *b);
+/* Tiles per round the schedules of later prk_batch_create calls aim for
+ * (process-wide tuning knob; never changes results). */
+int prk_set_batch_round_tiles(long long tiles);
 int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override, double *norms,
                          void *stream);
 

@@ -21,18 +21,19 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 int vec_of(int dt) { return dt == F64 ? 2 : dt == F32 ? 4 : 8; }
 
-// Rows per tile: short blocks keep the waves of a launch on a compact band
-// of rows; the 2R halo rows each block re-reads grow with the radius.
-int default_tile_rows(int radius) { return radius <= 2 ? 8 : radius == 3 ? 12 : 16; }
+// Rows per tile: each tile re-reads 2R halo rows and pays a pipeline
+// prologue, so batched tiles are taller than the single-grid row blocks
+// (32 measured best for the 4096-grid mix on MI355X).
+int default_tile_rows(int) { return 32; }
 
 // Tiles a round aims for: several waves per CU on all 256 CUs, so a round's
 // tail (its slowest waves) is a small part of it.
-constexpr long long kRoundTiles = 16384;
+long long g_round_tiles = 16384;
 
 struct Round {
-    size_t first_item;
-    int nitems;
-    int waves;
+    size_t first_work;  // first workgroup entry
+    int blocks;         // workgroups in the launch
+    long long tiles;
 };
 
 }  // namespace
@@ -48,15 +49,17 @@ struct prk_batch {
     size_t pool_bytes = 0;
     void *meta = nullptr;      // descriptors, items, partials, norms
     BGrid *d_grids = nullptr;
-    RItem *d_items = nullptr;
+    RWork *d_work = nullptr;
     double *d_norms = nullptr;
-    std::vector<Round> rounds[3];
+    // classes: storage type x pattern x radius (1..4)
+    static constexpr int kClasses = 3 * 2 * kBatchMaxRadius;
+    std::vector<Round> rounds[kClasses];
     long long total_tiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t capture = nullptr;
-    hipStream_t branch[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    hipStream_t branch[kClasses] = {};
+    hipEvent_t fork = nullptr, join[kClasses] = {};
     hipEvent_t done = nullptr;
     bool launched = false;
 };
@@ -84,7 +87,7 @@ int stage_tiles(const BGrid &g, int stage) { return stage == g.iters + 3 ? 1 : g
 // active solve; a solve leaves after its final-norm stage and the next one
 // in submission order takes its place.
 void schedule_rounds(const std::vector<int> &ids, const std::vector<BGrid> &grids, long long target,
-                     std::vector<RItem> &items, std::vector<Round> &rounds)
+                     std::vector<RWork> &work, std::vector<Round> &rounds)
 {
     std::vector<int> active;
     std::vector<int> stage(grids.size(), 0);
@@ -96,12 +99,13 @@ void schedule_rounds(const std::vector<int> &ids, const std::vector<BGrid> &grid
             in_flight += grids[ids[next]].ntiles;
             ++next;
         }
-        Round r{items.size(), 0, 0};
+        Round r{work.size(), 0, 0};
         std::vector<int> keep;
         for (int g : active) {
-            items.push_back(RItem{g, stage[g], r.waves});
-            r.waves += stage_tiles(grids[g], stage[g]);
-            ++r.nitems;
+            const int tiles = stage_tiles(grids[g], stage[g]);
+            for (int t0 = 0; t0 < tiles; t0 += kWavesPerBlock) work.push_back(RWork{g, stage[g], t0});
+            r.blocks += (tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+            r.tiles += tiles;
             if (++stage[g] <= grids[g].iters + 3) keep.push_back(g);
             else in_flight -= grids[g].ntiles;
         }
@@ -115,7 +119,7 @@ void batch_free(prk_batch *b)
     if (!b) return;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < prk_batch::kClasses; ++d) {
         if (b->branch[d]) (void)hipStreamDestroy(b->branch[d]);
         if (b->join[d]) (void)hipEventDestroy(b->join[d]);
     }
@@ -127,15 +131,33 @@ void batch_free(prk_batch *b)
     delete b;
 }
 
-int launch_round(int dt, const RoundArgs &a, hipStream_t s)
+int class_of(const prk_solve &s) { return (s.dtype * 2 + s.pattern) * kBatchMaxRadius + (s.radius - 1); }
+
+template <int DT, int P>
+int launch_round_pr(int radius, const RoundArgs &a, unsigned blocks, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((a.total + kWavesPerBlock - 1) / kWavesPerBlock);
-    switch (dt) {
-    case F64: hipLaunchKernelGGL(batch_round_kernel<F64>, dim3(blocks), dim3(kThreads), 0, s, a); break;
-    case F32: hipLaunchKernelGGL(batch_round_kernel<F32>, dim3(blocks), dim3(kThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL(batch_round_kernel<BF16>, dim3(blocks), dim3(kThreads), 0, s, a); break;
+    switch (radius) {
+    case 1: hipLaunchKernelGGL((batch_round_kernel<DT, P, 1>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((batch_round_kernel<DT, P, 2>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((batch_round_kernel<DT, P, 3>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((batch_round_kernel<DT, P, 4>), dim3(blocks), dim3(kThreads), 0, s, a); break;
     }
     return check_hip(hipGetLastError(), "batch round launch");
+}
+
+int launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s)
+{
+    const int radius = cls % kBatchMaxRadius + 1;
+    const int pattern = (cls / kBatchMaxRadius) % 2;
+    const int dt = cls / (2 * kBatchMaxRadius);
+    switch (dt * 2 + pattern) {
+    case 0: return launch_round_pr<F64, STAR>(radius, a, blocks, s);
+    case 1: return launch_round_pr<F64, GRID>(radius, a, blocks, s);
+    case 2: return launch_round_pr<F32, STAR>(radius, a, blocks, s);
+    case 3: return launch_round_pr<F32, GRID>(radius, a, blocks, s);
+    case 4: return launch_round_pr<BF16, STAR>(radius, a, blocks, s);
+    default: return launch_round_pr<BF16, GRID>(radius, a, blocks, s);
+    }
 }
 
 }  // namespace
@@ -194,23 +216,31 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
         g.out = (char *)g.buf1 + arr;
     }
 
-    // one schedule per storage type
-    std::vector<RItem> items;
-    for (int dt = 0; dt < 3; ++dt) {
-        std::vector<int> ids;
-        for (int i = 0; i < count; ++i)
-            if (solves[i].dtype == dt) ids.push_back(i);
-        if (!ids.empty()) schedule_rounds(ids, b->grids, kRoundTiles, items, b->rounds[dt]);
+    // one schedule per class; the classes share the round budget by work
+    std::vector<RWork> work;
+    std::vector<int> ids[prk_batch::kClasses];
+    std::vector<long long> cwork(prk_batch::kClasses, 0);
+    long long all_work = 0;
+    for (int i = 0; i < count; ++i) {
+        const int c = class_of(solves[i]);
+        ids[c].push_back(i);
+        const long long w = (long long)(solves[i].iterations + 1) * b->grids[i].ntiles;
+        cwork[c] += w;
+        all_work += w;
     }
-    for (int dt = 0; dt < 3; ++dt)
-        for (const Round &r : b->rounds[dt]) b->total_tiles += r.waves;
+    for (int c = 0; c < prk_batch::kClasses; ++c) {
+        if (ids[c].empty()) continue;
+        const long long target = std::max<long long>(1024, g_round_tiles * cwork[c] / std::max<long long>(all_work, 1));
+        schedule_rounds(ids[c], b->grids, target, work, b->rounds[c]);
+        for (const Round &r : b->rounds[c]) b->total_tiles += r.tiles;
+    }
 
     // metadata block
     size_t m = 0;
     const size_t o_grids = m;
     m = align_up(m + sizeof(BGrid) * count, 256);
-    const size_t o_items = m;
-    m = align_up(m + sizeof(RItem) * items.size(), 256);
+    const size_t o_work = m;
+    m = align_up(m + sizeof(RWork) * work.size(), 256);
     const size_t o_norms = m;
     m = align_up(m + sizeof(double) * count, 256);
     const size_t o_part = m;
@@ -222,7 +252,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
     }
     char *mb = (char *)b->meta;
     b->d_grids = (BGrid *)(mb + o_grids);
-    b->d_items = (RItem *)(mb + o_items);
+    b->d_work = (RWork *)(mb + o_work);
     b->d_norms = (double *)(mb + o_norms);
     DDv *parts = (DDv *)(mb + o_part);
     size_t pk = 0;
@@ -233,8 +263,8 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
     }
     rc = check_hip(hipMemcpy(b->d_grids, b->grids.data(), sizeof(BGrid) * count, hipMemcpyHostToDevice), "grids");
     if (!rc)
-        rc = check_hip(hipMemcpy(b->d_items, items.data(), sizeof(RItem) * items.size(), hipMemcpyHostToDevice),
-                       "items");
+        rc = check_hip(hipMemcpy(b->d_work, work.data(), sizeof(RWork) * work.size(), hipMemcpyHostToDevice),
+                       "work table");
     if (rc) {
         batch_free(b);
         return rc;
@@ -244,24 +274,24 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
     rc = check_hip(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "batch stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "batch event");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event");
-    for (int dt = 0; dt < 3 && !rc; ++dt) {
-        if (b->rounds[dt].empty()) continue;
-        rc = check_hip(hipStreamCreateWithFlags(&b->branch[dt], hipStreamNonBlocking), "batch branch stream");
-        if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->join[dt], hipEventDisableTiming), "batch event");
+    for (int c = 0; c < prk_batch::kClasses && !rc; ++c) {
+        if (b->rounds[c].empty()) continue;
+        rc = check_hip(hipStreamCreateWithFlags(&b->branch[c], hipStreamNonBlocking), "batch branch stream");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->join[c], hipEventDisableTiming), "batch event");
     }
     if (!rc) rc = check_hip(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "begin capture");
     if (!rc) {
         rc = check_hip(hipEventRecord(b->fork, b->capture), "fork");
-        for (int dt = 0; dt < 3 && !rc; ++dt) {
-            if (b->rounds[dt].empty()) continue;
-            rc = check_hip(hipStreamWaitEvent(b->branch[dt], b->fork, 0), "fork wait");
-            for (const Round &r : b->rounds[dt]) {
+        for (int c = 0; c < prk_batch::kClasses && !rc; ++c) {
+            if (b->rounds[c].empty()) continue;
+            rc = check_hip(hipStreamWaitEvent(b->branch[c], b->fork, 0), "fork wait");
+            for (const Round &r : b->rounds[c]) {
                 if (rc) break;
-                RoundArgs a{b->d_grids, b->d_items + r.first_item, r.nitems, r.waves};
-                rc = launch_round(dt, a, b->branch[dt]);
+                RoundArgs a{b->d_grids, b->d_work + r.first_work};
+                rc = launch_round(c, a, (unsigned)r.blocks, b->branch[c]);
             }
-            if (!rc) rc = check_hip(hipEventRecord(b->join[dt], b->branch[dt]), "join");
-            if (!rc) rc = check_hip(hipStreamWaitEvent(b->capture, b->join[dt], 0), "join wait");
+            if (!rc) rc = check_hip(hipEventRecord(b->join[c], b->branch[c]), "join");
+            if (!rc) rc = check_hip(hipStreamWaitEvent(b->capture, b->join[c], 0), "join wait");
         }
         hipGraph_t g = nullptr;
         const int rc2 = check_hip(hipStreamEndCapture(b->capture, &g), "end capture");
@@ -315,11 +345,22 @@ int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles
     if (!b) return set_error(PRK_EINVAL, "null batch");
     if (device_bytes) *device_bytes = (long long)b->pool_bytes;
     if (tiles) *tiles = b->total_tiles;
-    if (launches) *launches = (int)(b->rounds[0].size() + b->rounds[1].size() + b->rounds[2].size());
+    if (launches) {
+        size_t l = 0;
+        for (const auto &r : b->rounds) l += r.size();
+        *launches = (int)l;
+    }
     return PRK_OK;
 }
 
 void prk_batch_destroy(prk_batch *b) { batch_free(b); }
+
+int prk_set_batch_round_tiles(long long tiles)
+{
+    if (tiles < 1) return set_error(PRK_EINVAL, "round size must be >= 1 tile");
+    g_round_tiles = tiles;
+    return PRK_OK;
+}
 
 int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override, double *norms, void *stream)
 {

@@ -4,9 +4,12 @@
 // Two execution forms share the tile code below.
 //   Round form (batch_round_kernel, the one in use): the host cuts the
 //   schedule into rounds; a round holds at most one stage of each solve, so
-//   its tiles are independent and one plain launch runs them all; stage order
-//   within a solve comes from launch order (one round after the other, all
-//   captured in one hipGraph).  No wave ever waits on another.
+//   its tiles are independent and one plain launch runs them all (a table
+//   gives each workgroup its solve, stage and tiles); stage order within a
+//   solve comes from launch order.  Solves are grouped in classes of one
+//   (storage type, pattern, radius), each with its own kernel and schedule
+//   on its own branch of one captured hipGraph.  No wave ever waits on
+//   another.
 //   Persistent form (batch_kernel): one launch, waves pull tiles from per-XCD
 //   ticket queues and spin on per-solve counters.  It hung on MI355X and is
 //   disabled; the host refuses it (see prk_batch_create).
@@ -319,66 +322,48 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 }
 
 // ---------------------------------------------------------------------------
-// Round form.  One launch = one round: rows of (solve, stage) items with a
-// prefix of their tile counts; wave w of the launch runs tile w.  Stages:
-// 0 init, 1..K+1 sweeps, K+2 norm partials, K+3 the solve's final norm
-// (one tile: reduces the partials in tile order).
+// Round form.  One launch = one round.  Every workgroup of the launch has a
+// work entry (solve, stage, first tile) and its four waves run four
+// consecutive tiles of that stage.  Stages: 0 init, 1..K+1 sweeps, K+2 norm
+// partials, K+3 the solve's final norm (one tile: reduces the partials in
+// tile order).
 // ---------------------------------------------------------------------------
-struct RItem {
+struct RWork {
     int grid;
     int stage;
-    int first;  // first wave index of this item in the launch
+    int tile0;
 };
 
 struct RoundArgs {
     const BGrid *grids;
-    const RItem *items;
-    int nitems;
-    int total;  // waves in the launch
+    const RWork *work;  // one entry per workgroup
 };
 
-template <int DT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_round_kernel(RoundArgs a)
+// One kernel per (storage type, pattern, radius): each keeps the registers
+// and LDS ring its own stencil needs.
+template <int DT, int P, int R>
+__global__ __launch_bounds__(kThreads) void batch_round_kernel(RoundArgs a)
 {
-    typedef SweepGeom<DT, kBatchMaxRadius> GM;
-    typedef typename GM::T T;
-    constexpr int SW = GM::SW;
-    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][GM::S * GM::LR];
+    typedef SweepGeom<DT, R> G;
+    typedef typename G::T T;
+    constexpr int SW = G::SW;
+    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][G::S * G::LR];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * kWavesPerBlock + wave;
-    if (w >= a.total) return;
-    int lo = 0, hi = a.nitems - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.items[mid].first <= w) lo = mid;
-        else hi = mid - 1;
-    }
-    const RItem it = a.items[lo];
-    const BGrid &g = a.grids[it.grid];
-    const int tile = w - it.first;
+    const RWork wk = a.work[blockIdx.x];
+    const BGrid &g = a.grids[wk.grid];
+    const int norm_stage = g.iters + 2;
+    const int tile = wk.tile0 + wave;
+    if (tile >= (wk.stage == norm_stage + 1 ? 1 : g.ntiles)) return;
     const int rb = tile / g.strips;
     const int c0 = (tile - rb * g.strips) * SW;
     const int r0 = rb * g.rows;
     const int r1 = min(r0 + g.rows, g.n);
-    const int norm_stage = g.iters + 2;
-    if (it.stage == 0) {
+    if (wk.stage == 0) {
         init_tile<DT>(g, c0, r0, r1, lane);
-    } else if (it.stage < norm_stage) {
-        const int k = it.stage - 1;
-        T *ring = lds[wave];
-        switch (g.pattern * 8 + g.radius) {
-        case 1: sweep_tile<DT, STAR, 1>(g, k, c0, r0, r1, ring, lane); break;
-        case 2: sweep_tile<DT, STAR, 2>(g, k, c0, r0, r1, ring, lane); break;
-        case 3: sweep_tile<DT, STAR, 3>(g, k, c0, r0, r1, ring, lane); break;
-        case 4: sweep_tile<DT, STAR, 4>(g, k, c0, r0, r1, ring, lane); break;
-        case 9: sweep_tile<DT, GRID, 1>(g, k, c0, r0, r1, ring, lane); break;
-        case 10: sweep_tile<DT, GRID, 2>(g, k, c0, r0, r1, ring, lane); break;
-        case 11: sweep_tile<DT, GRID, 3>(g, k, c0, r0, r1, ring, lane); break;
-        case 12: sweep_tile<DT, GRID, 4>(g, k, c0, r0, r1, ring, lane); break;
-        default: break;
-        }
-    } else if (it.stage == norm_stage) {
+    } else if (wk.stage < norm_stage) {
+        sweep_tile<DT, P, R>(g, wk.stage - 1, c0, r0, r1, lds[wave], lane);
+    } else if (wk.stage == norm_stage) {
         const DDv part = norm_tile<DT>(g, c0, r0, r1, lane);
         if (lane == 0) g.partials[tile] = part;
     } else {
