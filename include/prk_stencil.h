@@ -218,6 +218,59 @@ void prk_batch_destroy(prk_batch *b);
 /* Tiles per round the schedules of later prk_batch_create calls aim for
  * (process-wide tuning knob; never changes results). */
 int prk_set_batch_round_tiles(long long tiles);
+
+/* ------------------------------------------------------------------------
+ * Continuous-batching server (no reference counterpart).  Solves are
+ * submitted at any time; a worker thread admits them first-come first-served
+ * into a device memory pool of budget_bytes, keeps their grids resident
+ * until they finish and runs every admitted request together in windows of
+ * rounds (one stage of each active request per round, one launch per
+ * storage-type/pattern/radius class per round, each window's launches
+ * captured into a hipGraph while the previous window runs).  Requests that
+ * arrive while a window runs join at the next one.
+ *   prk_server_submit   queue one solve; PRK_EBUSY if max_pending requests
+ *                       are already queued, PRK_ENOMEM if it can never fit
+ *                       the budget.  keep_output: hold the request's grids
+ *                       after it finishes until prk_server_release (for
+ *                       prk_server_fetch)
+ *   prk_server_poll     completed results (up to max_results)
+ *   prk_server_drain    wait until every accepted request is answered
+ *   prk_server_clock    the clock of the result timestamps (seconds)
+ * A result's norm is the L1 norm of the solve (as prk_l1norm); times are
+ * submission, admission and completion on prk_server_clock.
+ * ------------------------------------------------------------------------ */
+typedef struct prk_server prk_server;
+
+typedef struct prk_server_config {
+    long long budget_bytes; /* device memory for all admitted requests */
+    int max_pending;        /* queued (not yet admitted) requests beyond which submit refuses */
+    int max_active;         /* requests in flight at most */
+    long long round_tiles;  /* admit while the active requests' tiles per round stay below this */
+} prk_server_config;
+
+typedef struct prk_server_result {
+    uint64_t id;
+    double norm;
+    double t_submit, t_admit, t_done;
+    int status;
+} prk_server_result;
+
+typedef struct prk_server_stats {
+    long long submitted, served, rejected;
+    long long windows, launches;
+    long long pending, active;
+    long long pool_used, budget;
+} prk_server_stats;
+
+int prk_server_create(const prk_server_config *cfg, prk_server **out);
+int prk_server_submit(prk_server *s, const prk_solve *solve, int keep_output, uint64_t *id);
+int prk_server_poll(prk_server *s, prk_server_result *results, int max_results, int *count);
+int prk_server_drain(prk_server *s, double timeout_s);
+int prk_server_fetch(prk_server *s, uint64_t id, int which, void *host, size_t bytes);
+int prk_server_release(prk_server *s, uint64_t id);
+int prk_server_stats_get(prk_server *s, prk_server_stats *out);
+double prk_server_clock(void);
+void prk_server_destroy(prk_server *s);
 int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override, double *norms,
                          void *stream);
 

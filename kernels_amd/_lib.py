@@ -75,6 +75,36 @@ SIGNATURES.update({
 })
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", c_ll), ("max_pending", c_int), ("max_active", c_int), ("round_tiles", c_ll)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("id", c_u64), ("norm", c_double), ("t_submit", c_double), ("t_admit", c_double),
+                ("t_done", c_double), ("status", c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(f, c_ll) for f in ("submitted", "served", "rejected", "windows", "launches", "pending", "active",
+                                    "pool_used", "budget")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+SIGNATURES.update({
+    "prk_server_create": (c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_void_p)]),
+    "prk_server_submit": (c_int, [c_void_p, _SOLVE_P, c_int, ctypes.POINTER(c_u64)]),
+    "prk_server_poll": (c_int, [c_void_p, ctypes.POINTER(ServerResult), c_int, c_int_p]),
+    "prk_server_drain": (c_int, [c_void_p, c_double]),
+    "prk_server_fetch": (c_int, [c_void_p, c_u64, c_int, c_void_p, ctypes.c_size_t]),
+    "prk_server_release": (c_int, [c_void_p, c_u64]),
+    "prk_server_stats_get": (c_int, [c_void_p, ctypes.POINTER(ServerStats)]),
+    "prk_server_clock": (c_double, []),
+    "prk_server_destroy": (None, [c_void_p]),
+})
+
+
 class PrkError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"prk error {code}: {msg}")

@@ -131,7 +131,6 @@ void batch_free(prk_batch *b)
     delete b;
 }
 
-int class_of(const prk_solve &s) { return (s.dtype * 2 + s.pattern) * kBatchMaxRadius + (s.radius - 1); }
 
 template <int DT, int P>
 int launch_round_pr(int radius, const RoundArgs &a, unsigned blocks, hipStream_t s)
@@ -145,7 +144,13 @@ int launch_round_pr(int radius, const RoundArgs &a, unsigned blocks, hipStream_t
     return check_hip(hipGetLastError(), "batch round launch");
 }
 
-int launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s)
+}  // namespace
+
+namespace prk {
+
+int batch_class_of(const prk_solve &s) { return (s.dtype * 2 + s.pattern) * kBatchMaxRadius + (s.radius - 1); }
+
+int batch_launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s)
 {
     const int radius = cls % kBatchMaxRadius + 1;
     const int pattern = (cls / kBatchMaxRadius) % 2;
@@ -160,7 +165,11 @@ int launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s)
     }
 }
 
-}  // namespace
+int batch_validate_solve(const prk_solve &s) { return validate_solve(s); }
+
+}  // namespace prk
+
+using namespace prk;
 
 extern "C" {
 
@@ -222,7 +231,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
     std::vector<long long> cwork(prk_batch::kClasses, 0);
     long long all_work = 0;
     for (int i = 0; i < count; ++i) {
-        const int c = class_of(solves[i]);
+        const int c = batch_class_of(solves[i]);
         ids[c].push_back(i);
         const long long w = (long long)(solves[i].iterations + 1) * b->grids[i].ntiles;
         cwork[c] += w;
@@ -288,7 +297,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
             for (const Round &r : b->rounds[c]) {
                 if (rc) break;
                 RoundArgs a{b->d_grids, b->d_work + r.first_work};
-                rc = launch_round(c, a, (unsigned)r.blocks, b->branch[c]);
+                rc = batch_launch_round(c, a, (unsigned)r.blocks, b->branch[c]);
             }
             if (!rc) rc = check_hip(hipEventRecord(b->join[c], b->branch[c]), "join");
             if (!rc) rc = check_hip(hipStreamWaitEvent(b->capture, b->join[c], 0), "join wait");

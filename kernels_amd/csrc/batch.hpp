@@ -33,6 +33,7 @@
 // flag, skips its tile and lets every wave exit; the host reports it.
 #pragma once
 
+#include "prk_stencil.h"
 #include "sweep.hpp"
 
 namespace prk {
@@ -375,8 +376,14 @@ __global__ __launch_bounds__(kThreads) void batch_round_kernel(RoundArgs a)
     }
 }
 
+// Host helpers shared by the batch and the server (batch.hip).
+int batch_class_of(const prk_solve &s);  // (dtype, pattern, radius) class index
+constexpr int kBatchClasses = 3 * 2 * kBatchMaxRadius;
+int batch_launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s);
+int batch_validate_solve(const prk_solve &s);
+
 // Clears the per-launch counters (a kernel, so the launch can live in a graph).
-__global__ __launch_bounds__(256) void batch_reset_kernel(unsigned *tile_done, unsigned *norm_done, int ngrids,
+static __global__ __launch_bounds__(256) void batch_reset_kernel(unsigned *tile_done, unsigned *norm_done, int ngrids,
                                                           unsigned long long *tickets, int ntickets, int *error)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;

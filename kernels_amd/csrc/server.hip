@@ -1,0 +1,606 @@
+// Continuous-batching solve server.
+//
+// Requests (solves) arrive at any time through prk_server_submit.  A worker
+// thread admits them first-come first-served into one device memory pool
+// bounded by the HBM budget, keeps their grids resident until they finish,
+// and runs all admitted requests together in windows: a window is a few
+// rounds, a round holds the next stage of every active request (init, the
+// fused sweeps, norm partials, final norm -- the stages of the batched
+// path), and each round launches one tile kernel per (storage type,
+// pattern, radius) class over all of its requests.  A window's launches are
+// captured into a hipGraph on the host while the previous window runs on
+// the GPU.  Requests that arrive while a window runs join at the next one;
+// requests that finish free their memory for the queue behind them.  A
+// request's arithmetic is that of the single-grid sweep, so its result does
+// not depend on what it was batched with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.hpp"
+#include "prk_internal.hpp"
+
+namespace prk {
+namespace {
+
+double now_s()
+{
+    using clk = std::chrono::steady_clock;
+    return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int vec_of(int dt) { return dt == F64 ? 2 : dt == F32 ? 4 : 8; }
+
+// First-fit sub-allocator over one device allocation, with coalescing.
+class Pool {
+  public:
+    void init(size_t size) { free_[0] = size; }
+    bool alloc(size_t bytes, size_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t at = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0;
+};
+
+struct Req {
+    uint64_t id;
+    prk_solve s;
+    int keep;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    size_t off = 0, bytes = 0;
+    BGrid g{};
+    int stage = 0;       // next stage to schedule
+    int final_window = -1;
+    double norm = 0;
+    bool done = false;
+};
+
+constexpr int kRoundsPerWindow = 8;
+
+}  // namespace
+}  // namespace prk
+
+using namespace prk;
+
+struct prk_server {
+    prk_server_config cfg{};
+    int device = 0;
+    char *pool_base = nullptr;
+    Pool pool;
+    hipStream_t stream = nullptr, capture = nullptr;
+    hipStream_t branch[kBatchClasses] = {};
+    hipEvent_t fork = nullptr, join[kBatchClasses] = {};
+    hipEvent_t window_done[2] = {nullptr, nullptr};
+    // per-window device tables (double-buffered) and pinned host staging
+    void *d_tab[2] = {nullptr, nullptr};
+    size_t d_tab_cap[2] = {0, 0};
+    void *h_tab[2] = {nullptr, nullptr};
+    size_t h_tab_cap[2] = {0, 0};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_idle;
+    std::deque<Req *> pending;
+    std::vector<Req *> active;  // admitted, still has stages to schedule
+    std::vector<Req *> inflight[2];  // finishing in the window of that slot
+    std::unordered_map<uint64_t, Req *> kept;
+    std::vector<prk_server_result> completed;
+    uint64_t next_id = 1;
+    bool stop = false;
+    int error = PRK_OK;
+    char error_msg[256] = {0};
+    prk_server_stats st{};
+    std::thread worker;
+};
+
+namespace {
+
+int fail(prk_server *s, int rc)
+{
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (!s->error) {
+        s->error = rc;
+        std::snprintf(s->error_msg, sizeof s->error_msg, "%s", prk_last_error());
+    }
+    s->cv_idle.notify_all();
+    return rc;
+}
+
+size_t request_bytes(const prk_solve &s, BGrid &g)
+{
+    const int V = vec_of(s.dtype);
+    std::memset(&g, 0, sizeof g);
+    g.pitch = (long long)align_up((size_t)s.n, (size_t)V);
+    g.n = s.n;
+    g.pattern = s.pattern;
+    g.radius = s.radius;
+    g.iters = s.iterations;
+    g.rows = std::min(s.n, 32);
+    g.strips = (s.n + 64 * V - 1) / (64 * V);
+    g.ntiles = g.strips * ((s.n + g.rows - 1) / g.rows);
+    g.init_kind = s.init;
+    g.seed = s.seed;
+    const size_t arr = align_up((size_t)s.n * (size_t)g.pitch * dtype_size(s.dtype), 256);
+    return 3 * arr + align_up(sizeof(DDv) * (size_t)g.ntiles, 256) + 256;
+}
+
+void place_request(prk_server *s, Req *r)
+{
+    const size_t es = dtype_size(r->s.dtype);
+    const size_t arr = align_up((size_t)r->g.n * (size_t)r->g.pitch * es, 256);
+    char *base = s->pool_base + r->off;
+    r->g.buf0 = base;
+    r->g.buf1 = base + arr;
+    r->g.out = base + 2 * arr;
+    r->g.partials = (DDv *)(base + 3 * arr);
+    r->g.norm = (double *)(base + 3 * arr + align_up(sizeof(DDv) * (size_t)r->g.ntiles, 256));
+}
+
+int ensure_capacity(prk_server *s, int slot, size_t bytes)
+{
+    if (s->d_tab_cap[slot] < bytes) {
+        if (s->d_tab[slot]) (void)hipFree(s->d_tab[slot]);
+        s->d_tab[slot] = nullptr;
+        const size_t cap = std::max(bytes, s->d_tab_cap[slot] * 2);
+        int rc = check_hip(hipMalloc(&s->d_tab[slot], cap), "server tables");
+        if (rc) return rc;
+        s->d_tab_cap[slot] = cap;
+    }
+    if (s->h_tab_cap[slot] < bytes) {
+        if (s->h_tab[slot]) (void)hipHostFree(s->h_tab[slot]);
+        s->h_tab[slot] = nullptr;
+        const size_t cap = std::max(bytes, s->h_tab_cap[slot] * 2);
+        int rc = check_hip(hipHostMalloc(&s->h_tab[slot], cap, hipHostMallocDefault), "server staging");
+        if (rc) return rc;
+        s->h_tab_cap[slot] = cap;
+    }
+    return PRK_OK;
+}
+
+// Retire the requests whose final stage ran in the window of `slot`.
+void retire(prk_server *s, int slot)
+{
+    std::vector<Req *> fin;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        fin.swap(s->inflight[slot]);
+    }
+    const double t = now_s();
+    for (Req *r : fin) {
+        double norm = 0;
+        if (hipMemcpy(&norm, r->g.norm, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+            fail(s, set_error(PRK_EHIP, "server: norm copy failed"));
+        }
+        r->norm = norm;
+        r->t_done = t;
+        r->done = true;
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
+    for (Req *r : fin) {
+        prk_server_result res;
+        res.id = r->id;
+        res.norm = r->norm;
+        res.t_submit = r->t_submit;
+        res.t_admit = r->t_admit;
+        res.t_done = r->t_done;
+        res.status = PRK_OK;
+        s->completed.push_back(res);
+        s->st.served += 1;
+        if (r->keep) {
+            s->kept[r->id] = r;
+        } else {
+            s->pool.release(r->off, r->bytes);
+            delete r;
+        }
+    }
+    s->st.pool_used = (long long)s->pool.used();
+    s->cv_idle.notify_all();
+}
+
+// Admit pending requests first-come first-served while memory and the
+// active-work target allow (called with the lock held).
+void admit(prk_server *s)
+{
+    long long tiles = 0;
+    for (Req *r : s->active) tiles += r->g.ntiles;
+    const double t = now_s();
+    while (!s->pending.empty()) {
+        Req *r = s->pending.front();
+        if (!s->active.empty() && tiles >= s->cfg.round_tiles) break;
+        if ((int)s->active.size() >= s->cfg.max_active) break;
+        size_t off = 0;
+        if (!s->pool.alloc(r->bytes, &off)) break;  // head waits for memory
+        s->pending.pop_front();
+        r->off = off;
+        place_request(s, r);
+        r->t_admit = t;
+        s->active.push_back(r);
+        tiles += r->g.ntiles;
+    }
+    s->st.pool_used = (long long)s->pool.used();
+}
+
+// Build window `w` into table slot w%2 and capture its launches.
+int build_window(prk_server *s, long long w)
+{
+    const int slot = (int)(w & 1);
+    std::vector<Req *> act;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        act = s->active;
+    }
+    // grids table + one work list per (round, class)
+    std::vector<BGrid> grids(act.size());
+    for (size_t i = 0; i < act.size(); ++i) grids[i] = act[i]->g;
+    std::vector<RWork> work;
+    struct Launch {
+        int cls;
+        size_t first;
+        int blocks;
+    };
+    std::vector<Launch> launches[kBatchClasses];
+    std::vector<Req *> finishing;
+    for (int round = 0; round < kRoundsPerWindow; ++round) {
+        std::vector<RWork> per[kBatchClasses];
+        bool any = false;
+        for (size_t i = 0; i < act.size(); ++i) {
+            Req *r = act[i];
+            const int last = r->g.iters + 3;
+            if (r->stage > last) continue;
+            const int tiles = r->stage == last ? 1 : r->g.ntiles;
+            const int c = batch_class_of(r->s);
+            for (int t0 = 0; t0 < tiles; t0 += kWavesPerBlock) per[c].push_back(RWork{(int)i, r->stage, t0});
+            if (r->stage == last) {
+                r->final_window = (int)w;
+                finishing.push_back(r);
+            }
+            ++r->stage;
+            any = true;
+        }
+        if (!any) break;
+        for (int c = 0; c < kBatchClasses; ++c) {
+            if (per[c].empty()) continue;
+            launches[c].push_back(Launch{c, work.size(), (int)per[c].size()});
+            work.insert(work.end(), per[c].begin(), per[c].end());
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        std::vector<Req *> keep;
+        for (Req *r : s->active)
+            if (r->stage <= r->g.iters + 3) keep.push_back(r);
+        s->active.swap(keep);
+        s->inflight[slot] = finishing;
+    }
+    const size_t gbytes = align_up(sizeof(BGrid) * grids.size(), 256);
+    const size_t wbytes = sizeof(RWork) * work.size();
+    int rc = ensure_capacity(s, slot, gbytes + wbytes + 256);
+    if (rc) return rc;
+    std::memcpy(s->h_tab[slot], grids.data(), sizeof(BGrid) * grids.size());
+    std::memcpy((char *)s->h_tab[slot] + gbytes, work.data(), wbytes);
+    rc = check_hip(hipMemcpyAsync(s->d_tab[slot], s->h_tab[slot], gbytes + wbytes, hipMemcpyHostToDevice, s->stream),
+                   "server table upload");
+    if (rc) return rc;
+    const BGrid *d_grids = (const BGrid *)s->d_tab[slot];
+    const RWork *d_work = (const RWork *)((char *)s->d_tab[slot] + gbytes);
+
+    // capture: fork one branch per class, its rounds in order, join
+    if (s->exec[slot]) (void)hipGraphExecDestroy(s->exec[slot]);
+    if (s->graph[slot]) (void)hipGraphDestroy(s->graph[slot]);
+    s->exec[slot] = nullptr;
+    s->graph[slot] = nullptr;
+    rc = check_hip(hipStreamBeginCapture(s->capture, hipStreamCaptureModeThreadLocal), "server capture");
+    if (rc) return rc;
+    rc = check_hip(hipEventRecord(s->fork, s->capture), "server fork");
+    for (int c = 0; c < kBatchClasses && !rc; ++c) {
+        if (launches[c].empty()) continue;
+        rc = check_hip(hipStreamWaitEvent(s->branch[c], s->fork, 0), "server fork wait");
+        for (const Launch &l : launches[c]) {
+            if (rc) break;
+            RoundArgs a{d_grids, d_work + l.first};
+            rc = batch_launch_round(c, a, (unsigned)l.blocks, s->branch[c]);
+        }
+        if (!rc) rc = check_hip(hipEventRecord(s->join[c], s->branch[c]), "server join");
+        if (!rc) rc = check_hip(hipStreamWaitEvent(s->capture, s->join[c], 0), "server join wait");
+    }
+    hipGraph_t g = nullptr;
+    const int rc2 = check_hip(hipStreamEndCapture(s->capture, &g), "server end capture");
+    s->graph[slot] = g;
+    if (!rc) rc = rc2;
+    if (!rc) rc = check_hip(hipGraphInstantiate(&s->exec[slot], g, nullptr, nullptr, 0), "server instantiate");
+    if (!rc) rc = check_hip(hipGraphLaunch(s->exec[slot], s->stream), "server graph launch");
+    if (!rc) rc = check_hip(hipEventRecord(s->window_done[slot], s->stream), "server window event");
+    size_t nl = 0;
+    for (auto &v : launches) nl += v.size();
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->st.windows += 1;
+    s->st.launches += (long long)nl;
+    return rc;
+}
+
+void worker_main(prk_server *s)
+{
+    if (hipSetDevice(s->device) != hipSuccess) {
+        fail(s, set_error(PRK_EHIP, "server: hipSetDevice failed"));
+        return;
+    }
+    long long w = 0;
+    bool prev_inflight = false;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(s->mu);
+            admit(s);
+            if (s->active.empty() && !prev_inflight) {
+                if (s->stop) break;
+                s->cv_idle.notify_all();
+                if (s->pending.empty()) {
+                    s->cv_work.wait(lock, [&] { return s->stop || !s->pending.empty(); });
+                } else {
+                    // queued work that does not fit: kept outputs hold the
+                    // memory until released; look again shortly
+                    s->cv_work.wait_for(lock, std::chrono::milliseconds(5));
+                }
+                admit(s);
+                if (s->active.empty()) {
+                    if (s->stop && s->pending.empty()) break;
+                    continue;
+                }
+            }
+        }
+        bool launched = false;
+        {
+            std::unique_lock<std::mutex> lock(s->mu);
+            launched = !s->active.empty();
+        }
+        if (launched) {
+            if (build_window(s, w) != PRK_OK) {
+                fail(s, PRK_EHIP);
+                break;
+            }
+        }
+        // wait for the previous window, retire what finished in it
+        if (prev_inflight) {
+            const int pslot = (int)((w - 1) & 1);
+            if (hipEventSynchronize(s->window_done[pslot]) != hipSuccess) {
+                fail(s, set_error(PRK_EHIP, "server: window failed"));
+                break;
+            }
+            retire(s, pslot);
+        }
+        if (launched) {
+            prev_inflight = true;
+            ++w;
+        } else {
+            prev_inflight = false;
+        }
+    }
+    if (prev_inflight) {
+        const int pslot = (int)((w - 1) & 1);
+        (void)hipEventSynchronize(s->window_done[pslot]);
+        retire(s, pslot);
+    }
+}
+
+void server_free(prk_server *s)
+{
+    if (!s) return;
+    for (int k = 0; k < 2; ++k) {
+        if (s->exec[k]) (void)hipGraphExecDestroy(s->exec[k]);
+        if (s->graph[k]) (void)hipGraphDestroy(s->graph[k]);
+        if (s->window_done[k]) (void)hipEventDestroy(s->window_done[k]);
+        if (s->d_tab[k]) (void)hipFree(s->d_tab[k]);
+        if (s->h_tab[k]) (void)hipHostFree(s->h_tab[k]);
+    }
+    for (int c = 0; c < kBatchClasses; ++c) {
+        if (s->branch[c]) (void)hipStreamDestroy(s->branch[c]);
+        if (s->join[c]) (void)hipEventDestroy(s->join[c]);
+    }
+    if (s->fork) (void)hipEventDestroy(s->fork);
+    if (s->capture) (void)hipStreamDestroy(s->capture);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->pool_base) (void)hipFree(s->pool_base);
+    for (Req *r : s->pending) delete r;
+    for (Req *r : s->active) delete r;
+    for (auto &kv : s->kept) delete kv.second;
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prk_server_create(const prk_server_config *cfg, prk_server **out)
+{
+    if (!cfg || !out) return set_error(PRK_EINVAL, "null server config");
+    if (cfg->budget_bytes < (1 << 20) || cfg->max_pending < 1 || cfg->max_active < 1 || cfg->round_tiles < 1)
+        return set_error(PRK_EINVAL, "bad server config");
+    prk_server *s = new prk_server();
+    s->cfg = *cfg;
+    int rc = check_hip(hipGetDevice(&s->device), "hipGetDevice");
+    if (!rc) rc = check_hip(hipMalloc((void **)&s->pool_base, (size_t)cfg->budget_bytes), "server pool");
+    if (rc) {
+        server_free(s);
+        return rc == PRK_EHIP ? set_error(PRK_ENOMEM, "cannot reserve %lld bytes", cfg->budget_bytes) : rc;
+    }
+    s->pool.init((size_t)cfg->budget_bytes);
+    s->st.budget = cfg->budget_bytes;
+    rc = check_hip(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "server stream");
+    if (!rc) rc = check_hip(hipStreamCreateWithFlags(&s->capture, hipStreamNonBlocking), "server capture stream");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&s->fork, hipEventDisableTiming), "server event");
+    for (int c = 0; c < kBatchClasses && !rc; ++c) {
+        rc = check_hip(hipStreamCreateWithFlags(&s->branch[c], hipStreamNonBlocking), "server branch");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&s->join[c], hipEventDisableTiming), "server event");
+    }
+    for (int k = 0; k < 2 && !rc; ++k)
+        rc = check_hip(hipEventCreateWithFlags(&s->window_done[k], hipEventDisableTiming), "server event");
+    if (rc) {
+        server_free(s);
+        return rc;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return PRK_OK;
+}
+
+int prk_server_submit(prk_server *s, const prk_solve *solve, int keep_output, uint64_t *id)
+{
+    if (!s || !solve) return set_error(PRK_EINVAL, "null server or solve");
+    int rc = batch_validate_solve(*solve);
+    if (rc) return rc;
+    Req *r = new Req();
+    r->s = *solve;
+    r->keep = keep_output;
+    r->bytes = request_bytes(*solve, r->g);
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->error) {
+        delete r;
+        return set_error(s->error, "server failed: %s", s->error_msg);
+    }
+    if ((long long)r->bytes > s->cfg.budget_bytes) {
+        delete r;
+        s->st.rejected += 1;
+        return set_error(PRK_ENOMEM, "request needs %zu bytes, budget %lld", r->bytes, s->cfg.budget_bytes);
+    }
+    if ((int)s->pending.size() >= s->cfg.max_pending) {
+        delete r;
+        s->st.rejected += 1;
+        return set_error(PRK_EBUSY, "queue full (%d pending)", s->cfg.max_pending);
+    }
+    r->id = s->next_id++;
+    s->pending.push_back(r);
+    s->st.submitted += 1;
+    if (id) *id = r->id;
+    s->cv_work.notify_one();
+    return PRK_OK;
+}
+
+int prk_server_poll(prk_server *s, prk_server_result *results, int max_results, int *count)
+{
+    if (!s || !count || (max_results > 0 && !results)) return set_error(PRK_EINVAL, "bad poll arguments");
+    std::lock_guard<std::mutex> lock(s->mu);
+    const int n = std::min<int>(max_results, (int)s->completed.size());
+    for (int i = 0; i < n; ++i) results[i] = s->completed[i];
+    s->completed.erase(s->completed.begin(), s->completed.begin() + n);
+    *count = n;
+    if (s->error) return set_error(s->error, "server failed: %s", s->error_msg);
+    return PRK_OK;
+}
+
+int prk_server_drain(prk_server *s, double timeout_s)
+{
+    if (!s) return set_error(PRK_EINVAL, "null server");
+    std::unique_lock<std::mutex> lock(s->mu);
+    // every accepted request has been answered
+    const bool ok = s->cv_idle.wait_for(lock, std::chrono::duration<double>(timeout_s),
+                                        [&] { return s->error || s->st.served >= s->st.submitted; });
+    if (s->error) return set_error(s->error, "server failed: %s", s->error_msg);
+    return ok ? PRK_OK : set_error(PRK_ETIMEOUT, "server did not drain in %.1f s", timeout_s);
+}
+
+int prk_server_fetch(prk_server *s, uint64_t id, int which, void *host, size_t bytes)
+{
+    if (!s || !host || which < 0 || which > 1) return set_error(PRK_EINVAL, "bad fetch arguments");
+    Req *r = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        auto it = s->kept.find(id);
+        if (it == s->kept.end()) return set_error(PRK_ENOTFOUND, "no kept output for request %llu", (unsigned long long)id);
+        r = it->second;
+    }
+    const size_t es = dtype_size(r->s.dtype);
+    if (bytes < (size_t)r->g.n * r->g.n * es) return set_error(PRK_EINVAL, "host buffer too small");
+    const void *src = which == 0 ? r->g.out : ((r->s.iterations + 1) % 2 ? r->g.buf1 : r->g.buf0);
+    return check_hip(hipMemcpy2D(host, (size_t)r->g.n * es, src, (size_t)r->g.pitch * es, (size_t)r->g.n * es,
+                                 (size_t)r->g.n, hipMemcpyDeviceToHost),
+                     "server fetch");
+}
+
+int prk_server_release(prk_server *s, uint64_t id)
+{
+    if (!s) return set_error(PRK_EINVAL, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->kept.find(id);
+    if (it == s->kept.end()) return set_error(PRK_ENOTFOUND, "no kept output for request %llu", (unsigned long long)id);
+    s->pool.release(it->second->off, it->second->bytes);
+    delete it->second;
+    s->kept.erase(it);
+    s->st.pool_used = (long long)s->pool.used();
+    s->cv_work.notify_one();
+    return PRK_OK;
+}
+
+double prk_server_clock(void) { return now_s(); }
+
+int prk_server_stats_get(prk_server *s, prk_server_stats *out)
+{
+    if (!s || !out) return set_error(PRK_EINVAL, "bad stats arguments");
+    std::lock_guard<std::mutex> lock(s->mu);
+    *out = s->st;
+    out->pending = (long long)s->pending.size();
+    out->active = (long long)s->active.size();
+    return PRK_OK;
+}
+
+void prk_server_destroy(prk_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+        for (Req *r : s->pending) {
+            delete r;
+            s->st.rejected += 1;
+        }
+        s->pending.clear();
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    server_free(s);
+}
+
+}  // extern "C"

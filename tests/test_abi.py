@@ -128,3 +128,14 @@ def test_batch_persistent_form_is_refused():
     assert L.prk_batch_create(arr, 1, 0, 4, ctypes.byref(h)) == -1
     bad = (Solve * 1)(Solve(0, 5, 1, 64, 1, 1, 0))
     assert L.prk_batch_create(bad, 1, 0, 0, ctypes.byref(h)) == -2   # radius 5 not batched
+
+
+def test_server_config_rejected_before_device():
+    import ctypes
+    from kernels_amd._lib import ServerConfig
+    L = kernels_amd.lib()
+    h = ctypes.c_void_p()
+    for cfg in (ServerConfig(0, 1, 1, 1), ServerConfig(1 << 30, 0, 1, 1), ServerConfig(1 << 30, 1, 0, 1),
+                ServerConfig(1 << 30, 1, 1, 0)):
+        assert L.prk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
+    assert L.prk_server_clock() > 0

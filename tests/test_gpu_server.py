@@ -1,0 +1,111 @@
+"""The continuous-batching server: results against the oracle, the same
+alone and in a mix (bitwise), admission under a tight HBM budget, queue-full
+rejection, and requests arriving while windows run."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from kernels_amd.server import Server
+from tests._oracle import BF16, F32, F64
+from tests.test_gpu_batch import ref_solve
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    (0, 2, F64, 130, 3, 1, 21),
+    (1, 3, F32, 77, 9, 1, 22),
+    (1, 4, BF16, 300, 2, 1, 23),
+    (0, 1, F32, 700, 1, 0, 0),
+    (1, 2, F64, 9, 0, 1, 24),
+    (0, 4, F64, 257, 12, 0, 0),
+    (1, 1, BF16, 64, 6, 0, 0),
+    (0, 3, F32, 515, 2, 1, 25),
+    (1, 4, F32, 33, 17, 1, 26),
+    (0, 2, BF16, 96, 1, 1, 27),
+]
+
+
+def run_all(srv, specs, keep=True):
+    ids = [srv.submit(s, keep=keep) for s in specs]
+    srv.drain(120)
+    res = {r["id"]: r for r in srv.poll()}
+    assert set(res) == set(ids)
+    return ids, res
+
+
+def test_server_matches_oracle(gpu, oracle):
+    srv = Server(budget_bytes=1 << 30)
+    ids, res = run_all(srv, MIX)
+    for rid, s in zip(ids, MIX):
+        print(f"request {rid}: {s}", flush=True)
+        out, fin, norm = ref_solve(oracle, s)
+        assert np.array_equal(srv.fetch(rid).view(np.uint8), out.view(np.uint8)), s
+        assert np.array_equal(srv.fetch(rid, 1).view(np.uint8), fin.view(np.uint8)), s
+        assert abs(res[rid]["norm"] - norm) <= 1e-13 * abs(norm), s
+        r = res[rid]
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+        srv.release(rid)
+    st = srv.stats()
+    assert st["served"] == len(MIX) and st["pool_used"] == 0 and st["windows"] >= 1
+    srv.close()
+
+
+def test_alone_equals_in_mix(gpu):
+    mix = MIX * 2
+    srv = Server(budget_bytes=1 << 30)
+    ids, res = run_all(srv, mix)
+    mixed = {i: (srv.fetch(rid), res[rid]["norm"]) for i, rid in enumerate(ids)}
+    srv.close()
+    for i, s in enumerate(MIX):
+        alone = Server(budget_bytes=1 << 30)
+        aid, ares = run_all(alone, [s])
+        out = alone.fetch(aid[0])
+        for j in (i, i + len(MIX)):
+            assert np.array_equal(out.view(np.uint8), mixed[j][0].view(np.uint8)), s
+            assert ares[aid[0]]["norm"] == mixed[j][1], s
+        alone.close()
+
+
+def test_tight_budget_and_rejections(gpu, oracle):
+    # room for about two 300x300 FP64 requests at a time
+    srv = Server(budget_bytes=5 << 20, max_pending=64)
+    specs = [(i % 2, 1 + i % 4, F64, 300, 3 + i % 5, 1, 100 + i) for i in range(12)]
+    ids, res = run_all(srv, specs, keep=False)
+    for rid, s in zip(ids, specs):
+        _, _, norm = ref_solve(oracle, s)
+        assert abs(res[rid]["norm"] - norm) <= 1e-13 * abs(norm)
+    rid, rc = srv.try_submit((0, 2, F64, 4000, 1, 1, 0))   # can never fit 5 MiB
+    assert rid is None and rc == -4
+    srv.close()
+    small = Server(budget_bytes=1 << 20, max_pending=2)
+    codes = [small.try_submit((0, 1, F32, 200, 200, 1, k))[1] for k in range(8)]
+    assert -5 in codes and 0 in codes
+    small.drain(120)
+    assert small.stats()["rejected"] == codes.count(-5)
+    small.close()
+
+
+def test_arrivals_while_running(gpu, oracle):
+    srv = Server(budget_bytes=1 << 30)
+    specs = [(i % 2, 1 + i % 4, (F32, F64, BF16)[i % 3], 64 + 37 * i, 5 + (7 * i) % 40, 1, 200 + i)
+             for i in range(40)]
+    ids = []
+
+    def client():
+        for s in specs:
+            ids.append(srv.submit(s))
+            time.sleep(0.002)
+
+    th = threading.Thread(target=client)
+    th.start()
+    th.join()
+    srv.drain(120)
+    res = {r["id"]: r for r in srv.poll()}
+    assert len(res) == len(specs)
+    for rid, s in list(zip(ids, specs))[::5]:
+        _, _, norm = ref_solve(oracle, s)
+        assert abs(res[rid]["norm"] - norm) <= 1e-13 * abs(norm)
+    assert srv.stats()["windows"] >= 2
+    srv.close()
