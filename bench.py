@@ -40,6 +40,105 @@ WORKLOADS = {
 }
 DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 BATCH_WORKLOADS = ("batch4096_f32",)
+SERVE_WORKLOADS = ("serve",)
+SERVE_RATES = (500.0, 2000.0, 8000.0)
+
+
+def config5_requests(rate: float, seconds: float, seed: int = 11):
+    """Poisson arrivals at `rate` per second for `seconds`: n log-uniform in
+    [128, 8192], radius uniform in 1..4, star/grid 50/50, iterations uniform
+    in 1..200, storage 60% FP32 / 30% FP64 / 10% BF16, seeded U(-1,1) grids.
+    Returns (arrival offsets in s, solve specs)."""
+    import numpy as np
+    rng = np.random.default_rng([seed, int(rate)])
+    t, arrivals = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        arrivals.append(t)
+    m = len(arrivals)
+    ns = np.exp(rng.uniform(np.log(128), np.log(8192), m)).round().astype(int)
+    radii = rng.integers(1, 5, m)
+    pats = rng.integers(0, 2, m)
+    iters = rng.integers(1, 201, m)
+    dts = rng.choice([1, 0, 2], size=m, p=[0.6, 0.3, 0.1])
+    specs = [(int(p), int(r), int(d), int(n), int(k), 1, 1000003 * int(rate) + i)
+             for i, (p, r, d, n, k) in enumerate(zip(pats, radii, dts, ns, iters))]
+    return arrivals, specs
+
+
+def bench_serve(a) -> int:
+    """Open-loop serving run per offered rate: a client submits at the Poisson
+    arrival times; latency is completion minus scheduled arrival."""
+    import numpy as np
+    import torch
+    from kernels_amd.server import Server
+
+    torch.cuda.set_device(0)
+    lines = []
+    for rate in a.serve_rates:
+        arrivals, specs = config5_requests(rate, a.serve_seconds)
+        srv = Server(budget_bytes=int(a.serve_budget_gb * (1 << 30)), max_pending=a.serve_max_pending)
+        t0 = Server.clock() + 0.05
+        ids, rejected = {}, 0
+        for off, spec in zip(arrivals, specs):
+            target = t0 + off
+            while True:
+                d = target - Server.clock()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.001))
+            rid, rc = srv.try_submit(spec)
+            if rid is None:
+                rejected += 1
+            else:
+                ids[rid] = (target, spec)
+        t_offer_end = Server.clock()
+        srv.drain(600)
+        res = srv.poll(len(ids) + 16)
+        st = srv.stats()
+        srv.close()
+        torch.cuda.empty_cache()
+        lat = np.array([r["t_done"] - ids[r["id"]][0] for r in res])
+        qwait = np.array([r["t_admit"] - ids[r["id"]][0] for r in res])
+        t_end = max(r["t_done"] for r in res)
+        served_bytes = sum(solve_bytes(ids[r["id"]][1]) for r in res)
+        span = t_end - t0
+        lines.append({
+            "offered_per_s": rate, "offered": len(specs), "accepted": len(ids), "rejected": rejected,
+            "served": len(res), "served_per_s": round(len(res) / span, 1),
+            "served_per_s_during_offer": round(sum(1 for r in res if r["t_done"] <= t_offer_end)
+                                              / (t_offer_end - t0), 1),
+            "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+            "mean_s": round(float(lat.mean()), 3), "queue_wait_p50_s": round(float(np.percentile(qwait, 50)), 3),
+            "effective_gbs": round(served_bytes / span / 1e9, 1), "windows": st["windows"],
+            "launches": st["launches"], "host_build_s": round(st["build_seconds"], 2),
+            "host_wait_s": round(st["wait_seconds"], 2), "span_s": round(span, 2),
+        })
+        print(f"# rate {rate}: {json.dumps(lines[-1])}", file=sys.stderr, flush=True)
+    best = max(lines, key=lambda x: x["served_per_s"])
+    res = {
+        "metric": METRIC,
+        "value": best["served_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(lines),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed (60% f32, 30% f64, 10% bf16)",
+        "data": "synthetic (Poisson arrivals, seeded U(-1,1) grids, generator seed 11)",
+        "config": {"workload": "served request stream: n log-uniform [128,8192], R 1..4, star/grid, "
+                               "iterations 1..200, continuous batching under an HBM budget",
+                   "seconds_per_rate": a.serve_seconds, "budget_gb": a.serve_budget_gb,
+                   "max_pending": a.serve_max_pending},
+        "rates": lines,
+    }
+    print(json.dumps(res))
+    return 0
 
 
 def config4_solves(seed: int = 7, count: int = 4096):
@@ -201,7 +300,12 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="star2_f64_n30000", choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS))
+    ap.add_argument("--workload", default="star2_f64_n30000",
+                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS))
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=list(SERVE_RATES))
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-max-pending", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-peak", action="store_true", help="skip the copy/triad peak measurement")
     ap.add_argument("--no-tune", action="store_true",
@@ -221,6 +325,16 @@ def main(argv=None) -> int:
 
     if a.workload in BATCH_WORKLOADS:
         return bench_batch(a)
+    if a.workload in SERVE_WORKLOADS:
+        # The first run of this open-loop serving benchmark on MI355X (500
+        # req/s for 10 s, 200 GiB pool) ended in a host segmentation fault
+        # with no output.  The server's host logic runs clean under
+        # AddressSanitizer and ThreadSanitizer on the CPU with a recording HIP
+        # stand-in at the same arrival rates, so the cause is not found; the
+        # workload stays switched off until it is.
+        print("the serving benchmark is switched off: its first run on MI355X ended in a host "
+              "segmentation fault whose cause has not been found", file=sys.stderr)
+        return 2
 
     import torch
     import kernels_amd as K

@@ -260,6 +260,8 @@ typedef struct prk_server_stats {
     long long windows, launches;
     long long pending, active;
     long long pool_used, budget;
+    double build_seconds; /* host time spent building and capturing windows */
+    double wait_seconds;  /* host time spent waiting for windows to finish */
 } prk_server_stats;
 
 int prk_server_create(const prk_server_config *cfg, prk_server **out);

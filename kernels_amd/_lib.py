@@ -86,7 +86,7 @@ class ServerResult(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(f, c_ll) for f in ("submitted", "served", "rejected", "windows", "launches", "pending", "active",
-                                    "pool_used", "budget")]
+                                    "pool_used", "budget")] + [("build_seconds", c_double), ("wait_seconds", c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

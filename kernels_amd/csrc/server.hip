@@ -267,6 +267,7 @@ void admit(prk_server *s)
 // Build window `w` into table slot w%2 and capture its launches.
 int build_window(prk_server *s, long long w)
 {
+    const double t_build = now_s();
     const int slot = (int)(w & 1);
     std::vector<Req *> act;
     {
@@ -359,6 +360,7 @@ int build_window(prk_server *s, long long w)
     std::lock_guard<std::mutex> lock(s->mu);
     s->st.windows += 1;
     s->st.launches += (long long)nl;
+    s->st.build_seconds += now_s() - t_build;
     return rc;
 }
 
@@ -405,9 +407,14 @@ void worker_main(prk_server *s)
         // wait for the previous window, retire what finished in it
         if (prev_inflight) {
             const int pslot = (int)((w - 1) & 1);
+            const double t_wait = now_s();
             if (hipEventSynchronize(s->window_done[pslot]) != hipSuccess) {
                 fail(s, set_error(PRK_EHIP, "server: window failed"));
                 break;
+            }
+            {
+                std::lock_guard<std::mutex> lock(s->mu);
+                s->st.wait_seconds += now_s() - t_wait;
             }
             retire(s, pslot);
         }
