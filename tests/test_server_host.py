@@ -1,0 +1,48 @@
+"""The solve server's host logic (kernels_amd/csrc/server.hip) on the CPU:
+built host-only against a stand-in for the HIP runtime (tests/host/mock_hip.cc)
+with AddressSanitizer and with ThreadSanitizer, and driven from a client
+thread at serving rates (tests/host/server_driver.cc).  The stand-in replays
+each captured window and checks every work entry against its grid table, so
+admission, pool placement, window building and retirement are checked
+without a GPU."""
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CLANG = "/opt/rocm/llvm/bin/clang++"
+SOURCES = [ROOT / "kernels_amd/csrc/server.hip", ROOT / "tests/host/mock_hip.cc", ROOT / "tests/host/server_driver.cc"]
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(HIPCC) and os.path.exists(CLANG)), reason="needs hipcc")
+
+
+def build(tmp: Path, sanitizer: str) -> Path:
+    objs = []
+    for src in SOURCES:
+        obj = tmp / (src.name + ".o")
+        subprocess.run([HIPCC, "--offload-host-only", "-O1", "-g", "-std=c++17", "-fno-omit-frame-pointer",
+                        f"-I{ROOT / 'include'}", f"-I{ROOT / 'kernels_amd/csrc'}",
+                        "-Xarch_host", f"-fsanitize={sanitizer}", "-x", "hip", "-c", str(src), "-o", str(obj)],
+                       check=True, capture_output=True, text=True)
+        objs.append(str(obj))
+    exe = tmp / f"server_driver_{sanitizer}"
+    subprocess.run([CLANG, f"-fsanitize={sanitizer}", "-o", str(exe), *objs, "-lpthread"], check=True,
+                   capture_output=True, text=True)
+    return exe
+
+
+@pytest.mark.parametrize("sanitizer,args", [("address", ["500", "2", "200", "0"]),
+                                            ("address", ["100", "1", "200", "1"]),
+                                            ("thread", ["200", "1", "200", "0"])])
+def test_server_host_logic(tmp_path, sanitizer, args):
+    exe = build(tmp_path, sanitizer)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=300, env=env)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr[-4000:]
+    assert "failures 0" in r.stdout
+    assert "WARNING: ThreadSanitizer" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
