@@ -14,9 +14,11 @@
 // rows in LDS, and every tap of the output row R rows above is then read
 // from LDS.  Rows are software-pipelined: the loads of the next rows are in
 // flight while the current row is computed.  A workgroup is four waves on
-// adjacent strips; workgroups are mapped so that the eight XCDs each take a
-// contiguous run of tiles in column-major order, which keeps a tile's
-// vertical neighbours (whose rows it re-reads as halo) in the same L2.
+// adjacent strips.  The workgroup -> tile order is a launch parameter
+// (tile_of): row-major over the grid measured best on MI355X; the orders
+// that give each of the eight XCDs a contiguous run of tiles, so that a
+// tile's vertical neighbours (whose rows it re-reads as halo) share its L2,
+// are kept as alternatives for the autotuner and for other shapes.
 //
 // Arithmetic.  FP64: the products are summed in the generator's order and
 // the sum is added to out last, with contraction off, so results are

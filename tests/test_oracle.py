@@ -189,6 +189,15 @@ def test_lp_forms_close_to_f64(oracle):
     # analytic init, star 2: dyadic weights and integer data -> exact in FP32
     f32a, _ = oracle.run_lp(F32, 0, 2, n, K, 0)
     assert np.all(f32a[inner] == 2.0 * (K + 1))
+    # BF16, star 1, analytic: in = i + j + k stays below 2^8 (exact in an
+    # 8-bit significand), the weights are +-1/2 and every partial sum is a
+    # small integer, so every interior point is 2(K+1) and in ends at
+    # i + j + K + 1 exactly
+    m = 40
+    b16a, b16in = oracle.run_lp(BF16, 0, 1, m, K, 0)
+    assert np.all(bf16_bits_to_f64(b16a)[1:-1, 1:-1] == 2.0 * (K + 1))
+    i = np.arange(m, dtype=np.float64)
+    assert np.array_equal(bf16_bits_to_f64(b16in), i[:, None] + i[None, :] + K + 1)
 
 
 def test_lp_band_equals_full(oracle):
