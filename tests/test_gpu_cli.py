@@ -1,0 +1,57 @@
+"""The prk-stencil driver on the GPU: the reference's command line and report
+(Cxx11/stencil-hip.cc:97-149,246-264), the fused and unfused structures, the
+storage types, and the unsupported-radius path."""
+import os
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+REPO = Path(__file__).resolve().parents[1]
+
+pytestmark = pytest.mark.gpu
+EXE = REPO / "kernels_amd" / "prk-stencil"
+
+
+def run(args, **env):
+    if not EXE.exists():
+        pytest.fail("prk-stencil not built (python -m kernels_amd.build)")
+    print("prk-stencil", *args, env, flush=True)
+    r = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True, timeout=120,
+                       env={**os.environ, **env})
+    return r.returncode, r.stdout
+
+
+@pytest.mark.parametrize("args,env", [
+    ((10, 1000), {}),                                        # the reference's CPU case, default tile/star/R=2
+    ((10, 1000, 32, "star", 2), {"PRK_MODE": "unfused"}),    # stencil + add per iteration, device sync
+    ((10, 1000, 32, "grid", 3), {}),
+    ((10, 1000, 0, "star", 5), {}),                          # tile <= 0 -> n
+    ((10, 1000, 32, "star", 2), {"PRK_DTYPE": "f32"}),
+    ((4, 777, 32, "grid", 1), {"PRK_DTYPE": "f32"}),         # ragged n
+])
+def test_cli_validates(gpu, args, env):
+    rc, out = run(args, **env)
+    assert rc == 0, out
+    assert "Solution validates" in out
+    m = re.search(r"Rate \(MFlops/s\): ([0-9.eE+-]+) Avg time \(s\): ([0-9.eE+-]+)", out)
+    assert m and float(m.group(1)) > 0 and float(m.group(2)) > 0, out
+    assert f"Number of iterations = {args[0]}" in out and f"Grid size            = {args[1]}" in out
+
+
+def test_cli_seeded_norm_matches_oracle(gpu, oracle):
+    n, k = 300, 6
+    rc, out = run((k, n, 32, "grid", 2), PRK_INIT="seeded", PRK_SEED="42")
+    assert rc == 0, out
+    norm = float(re.search(r"L1 norm = ([0-9.eE+-]+)", out).group(1))
+    ref_out, _ = oracle.run(1, 2, n, 32, k, 1, 42)
+    ref = oracle.l1norm(n, 2, ref_out)
+    assert abs(norm - ref) <= 1e-5 * abs(ref), (norm, ref)
+
+
+def test_cli_unsupported_radius_fails(gpu):
+    # the reference selects its no-op operator and then fails validation;
+    # the library refuses the operator and the driver exits with 1
+    rc, out = run((5, 100, 32, "star", 6))
+    assert rc == 1 and "ERROR" in out, out
