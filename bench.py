@@ -374,7 +374,10 @@ def main(argv=None) -> int:
     sweeps = a.warmup + a.steps
     norm = K.l1norm(dtype, radius, out, stream)
     eps = {0: 1e-8, 1: 1e-4, 2: 1e-2}[dtype]
-    validated = abs(norm - 2.0 * sweeps) <= eps
+    # BF16 (8-bit significand) cannot hold i + j + k beyond 256, so the
+    # analytic closed form does not apply to it; its parity is checked
+    # bitwise against the oracle's BF16 arithmetic in the tests instead.
+    validated = None if dtype == 2 else abs(norm - 2.0 * sweeps) <= eps
     del bufs, out
     torch.cuda.empty_cache()
     if a.copy_calibration:
@@ -411,7 +414,7 @@ def main(argv=None) -> int:
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
                      "algorithmic_bytes_per_launch": bytes_iter},
         "tuning": {**tuning.as_dict(), "autotuned": not a.no_tune, "tune_s": round(t_tune, 2)},
-        "validated": bool(validated),
+        "validated": validated if validated is None else bool(validated),
         "norm": norm,
         "mflops_reference_formula": round(K.nominal_flops(pattern, radius, n) / (ms_step * 1e-3) / 1e6, 1),
         "points_per_s": (n - 2 * radius) ** 2 / (ms_step * 1e-3),
@@ -423,7 +426,7 @@ def main(argv=None) -> int:
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pattern, radius, n) if dtype == 0 else None
     print(json.dumps(res))
-    return 0 if validated else 1
+    return 1 if validated is False else 0
 
 
 if __name__ == "__main__":

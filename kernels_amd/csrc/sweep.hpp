@@ -119,12 +119,29 @@ __device__ inline float to_f32<float>(float x) { return x; }
 template <>
 __device__ inline float to_f32<bf16_t>(bf16_t x) { return bf16_to_f32(x); }
 
-template <typename T>
-__device__ inline T from_f32(float x);
-template <>
-__device__ inline float from_f32<float>(float x) { return x; }
-template <>
-__device__ inline bf16_t from_f32<bf16_t>(float x) { return f32_to_bf16(x); }
+// V floats -> V storage values.  BF16 pairs go through gfx950's packed
+// conversion (v_cvt_pk_bf16_f32), which rounds to nearest even like
+// f32_to_bf16 for every finite value.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+template <int V>
+__device__ inline void from_f32_vec(const float (&f)[V], float *y)
+{
+#pragma unroll
+    for (int v = 0; v < V; ++v) y[v] = f[v];
+}
+template <int V>
+__device__ inline void from_f32_vec(const float (&f)[V], bf16_t *y)
+{
+#pragma unroll
+    for (int v = 0; v < V; v += 2) {
+        const float hi = v + 1 < V ? f[v + 1] : f[v];
+        const uint32_t w = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){f[v], hi}, bf16x2_t));
+        y[v] = (bf16_t)w;
+        if (v + 1 < V) y[v + 1] = (bf16_t)(w >> 16);
+    }
+}
 
 // One row of input (main vector + halo vector) and one row of out, in flight.
 template <typename T, int V>
@@ -145,8 +162,10 @@ __device__ inline void plus_one(const typename DTypeTraits<DT>::T *x, typename D
         for (int v = 0; v < V; ++v) y[v] = x[v] + 1.0;
     } else {
         typedef typename DTypeTraits<DT>::T T;
+        float f[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) y[v] = from_f32<T>(to_f32<T>(x[v]) + 1.0f);
+        for (int v = 0; v < V; ++v) f[v] = to_f32<T>(x[v]) + 1.0f;
+        from_f32_vec<V>(f, y);
     }
 }
 
@@ -215,11 +234,13 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
         float win[2 * R + 1][V + 2 * HP];
 #pragma unroll
         for (int k = 0; k < 2 * R + 1; ++k) load_window<T, V, HP>(rows[k], win[k]);
+        float f[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const float acc = pair_sum_f32<P, R, V, HP>(win, v, std::make_index_sequence<St::pairs.count - 1>{});
-            res[v] = from_f32<T>(to_f32<T>(outv[v]) + acc);
+            f[v] = to_f32<T>(outv[v]) + acc;
         }
+        from_f32_vec<V>(f, res);
     }
 }
 
