@@ -232,6 +232,64 @@ def bench_batch(a) -> int:
     return 0 if rel32 < 1e-12 else 1
 
 
+TABLE_WORKLOADS = ("sweep_table_n16384",)
+
+
+def bench_table(a) -> int:
+    """Every stencil of the radius/precision sweep at n=16384 (star R=1..5,
+    grid R=1..3; FP64, FP32 and BF16 storage): autotune, then time
+    min(steps, 20) fused sweeps after one warm-up sweep."""
+    import torch
+    import kernels_amd as K
+
+    n = 16384
+    steps = min(a.steps, 20)
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    cases = []
+    for dtype in (0, 1, 2):
+        bufs = [K.alloc_grid(n, dtype), K.alloc_grid(n, dtype)]
+        out = K.alloc_grid(n, dtype)
+        for pattern, radius in [(0, r) for r in range(1, 6)] + [(1, r) for r in range(1, 4)]:
+            tuning = K.Tuning()
+            K.check(K.lib().prk_autotune(pattern, radius, dtype, n, 2, tuning))
+            K.init(K.INIT_ANALYTIC, 0, dtype, bufs[0], out, stream)
+            cur = 0
+            K.sweep(pattern, radius, dtype, bufs[cur], bufs[cur ^ 1], out, stream)
+            cur ^= 1
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                K.sweep(pattern, radius, dtype, bufs[cur], bufs[cur ^ 1], out, stream)
+                cur ^= 1
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            norm = K.l1norm(dtype, radius, out, stream)
+            gbs = K.stencil_bytes_per_iter(n, radius, dtype) / (ms * 1e-3) / 1e9
+            case = {"stencil": f"{'star' if pattern == 0 else 'grid'}{radius}", "dtype": DT_NAME[dtype],
+                    "ms_per_sweep": round(ms, 4), "gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4),
+                    "points_per_s": round((n - 2 * radius) ** 2 / (ms * 1e-3), 0),
+                    "mflops_reference_formula": round(K.nominal_flops(pattern, radius, n) / (ms * 1e-3) / 1e6, 1),
+                    "tuning": {k: v for k, v in tuning.as_dict().items() if k != "gbs"},
+                    "validated": None if dtype == 2 else abs(norm - 2.0 * (steps + 1)) <= (1e-8 if dtype == 0 else 1e-4)}
+            cases.append(case)
+            print(f"# {json.dumps(case)}", file=sys.stderr, flush=True)
+        del bufs, out
+        torch.cuda.empty_cache()
+    head = next(c for c in cases if c["stencil"] == "star2" and c["dtype"] == "f64")
+    res = {
+        "metric": METRIC, "value": head["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": head["ms_per_sweep"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (reference analytic init in[i][j]=i+j, out=0)",
+        "config": {"workload": "radius/precision table: n=16384 fused sweeps, star R=1..5 and grid R=1..3, "
+                               "FP64/FP32/BF16 storage, each autotuned"},
+        "cases": cases,
+    }
+    print(json.dumps(res))
+    return 0 if all(c["validated"] is not False for c in cases) else 1
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE, scaled by the copy-kernel calibration (profiles/)."""
@@ -301,7 +359,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="star2_f64_n30000",
-                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS))
+                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS) + list(TABLE_WORKLOADS))
     ap.add_argument("--serve-rates", type=float, nargs="+", default=list(SERVE_RATES))
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -325,6 +383,8 @@ def main(argv=None) -> int:
 
     if a.workload in BATCH_WORKLOADS:
         return bench_batch(a)
+    if a.workload in TABLE_WORKLOADS:
+        return bench_table(a)
     if a.workload in SERVE_WORKLOADS:
         # The first run of this open-loop serving benchmark on MI355X (500
         # req/s for 10 s, 200 GiB pool) ended in a host segmentation fault
