@@ -232,13 +232,18 @@ def bench_batch(a) -> int:
     return 0 if rel32 < 1e-12 else 1
 
 
-TABLE_WORKLOADS = ("sweep_table_n16384",)
+TABLE_WORKLOADS = {
+    # the radius/precision sweep's stencils, and the radii beyond the reference's table
+    "sweep_table_n16384": [(0, r) for r in range(1, 6)] + [(1, r) for r in range(1, 4)],
+    "sweep_table_ext_n16384": [(0, r) for r in range(6, 9)] + [(1, r) for r in range(4, 9)],
+}
 
 
 def bench_table(a) -> int:
-    """Every stencil of the radius/precision sweep at n=16384 (star R=1..5,
-    grid R=1..3; FP64, FP32 and BF16 storage): autotune, then time
-    min(steps, 20) fused sweeps after one warm-up sweep."""
+    """Stencils at n=16384 in FP64, FP32 and BF16 storage (the radius/
+    precision sweep's star R=1..5 and grid R=1..3, or the radii beyond the
+    reference's table): autotune, then time min(steps, 20) fused sweeps
+    after one warm-up sweep."""
     import torch
     import kernels_amd as K
 
@@ -250,7 +255,7 @@ def bench_table(a) -> int:
     for dtype in (0, 1, 2):
         bufs = [K.alloc_grid(n, dtype), K.alloc_grid(n, dtype)]
         out = K.alloc_grid(n, dtype)
-        for pattern, radius in [(0, r) for r in range(1, 6)] + [(1, r) for r in range(1, 4)]:
+        for pattern, radius in TABLE_WORKLOADS[a.workload]:
             tuning = K.Tuning()
             K.check(K.lib().prk_autotune(pattern, radius, dtype, n, 2, tuning))
             K.init(K.INIT_ANALYTIC, 0, dtype, bufs[0], out, stream)
@@ -277,12 +282,13 @@ def bench_table(a) -> int:
             print(f"# {json.dumps(case)}", file=sys.stderr, flush=True)
         del bufs, out
         torch.cuda.empty_cache()
-    head = next(c for c in cases if c["stencil"] == "star2" and c["dtype"] == "f64")
+    head = cases[0]
     res = {
         "metric": METRIC, "value": head["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 1,
         "ms_per_step": head["ms_per_sweep"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (reference analytic init in[i][j]=i+j, out=0)",
-        "config": {"workload": "radius/precision table: n=16384 fused sweeps, star R=1..5 and grid R=1..3, "
+        "config": {"workload": f"{a.workload}: n=16384 fused sweeps of "
+                               f"{', '.join(c['stencil'] for c in cases if c['dtype'] == 'f64')}, "
                                "FP64/FP32/BF16 storage, each autotuned"},
         "cases": cases,
     }

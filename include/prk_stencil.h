@@ -79,8 +79,10 @@ int prk_grid4(int n, int t, const double *in, double *out);
 int prk_grid5(int n, int t, const double *in, double *out);
 
 /* The (pattern, radius) switch of Cxx11/stencil.cc:142-159 as one call.
- * A radius outside 1..5 returns PRK_ENOSTENCIL (the reference aborts on the
- * CPU and silently does nothing on HIP). */
+ * Radii 1..5 are the reference's operators; 6..8 extend the table with the
+ * generator's own weight formulas (Cxx11/generate-cxx-stencil.py:209-232).
+ * A radius outside 1..8 returns PRK_ENOSTENCIL (the reference aborts on the
+ * CPU and silently does nothing on HIP beyond its 5). */
 int prk_stencil(int pattern, int radius, int n, int t, const double *in, double *out);
 
 /* in[i*n+j] += 1 on all n*n points.  Replaces the HIP `add` kernel,

@@ -160,6 +160,8 @@ int prk_autotune(int pattern, int radius, int dtype, int n, int reps, prk_tuning
     top.gbs = -1.0;
     int cur = 0;
     for (int v : kCandVariants) {
+        // beyond the reference's radii one variant is built (see launch_pr)
+        if (radius > kRefMaxRadius && v != 4) continue;
         for (int w : kCandWgs) {
             for (int rows : kCandRows) {
                 if (rc != PRK_OK) break;

@@ -10,7 +10,11 @@ namespace prk {
 enum Pattern : int { STAR = 0, GRID = 1 };
 enum DType : int { F64 = 0, F32 = 1, BF16 = 2 };
 
-constexpr int kMaxRadius = 5;
+// Radii 1..5 are the reference's generated operators (Cxx11/stencil.cc:142-159);
+// 6..8 follow the same weight formulas (the generator's loop bound, not its
+// formulas, stops at 5: Cxx11/generate-cxx-stencil.py:245).
+constexpr int kMaxRadius = 8;
+constexpr int kRefMaxRadius = 5;
 
 // ---------------------------------------------------------------------------
 // Weights.  Cxx11/generate-cxx-stencil.py:209-232 assigns W[a+R][b+R], where a

@@ -169,11 +169,36 @@ __device__ inline void plus_one(const typename DTypeTraits<DT>::T *x, typename D
     }
 }
 
-// Strict-order FP64 sum of the taps for output column v (fold over the
-// compile-time tap list, so every index and weight is an immediate).
+// Strict-order FP64 sums of the taps for the lane's V output columns (folds
+// over the compile-time tap list, so every index and weight is an
+// immediate).  Each chain keeps the generator's order.  Up to R = 2 the
+// chains run one column after the other; from R = 3 on the V chains advance
+// together, tap by tap, which the compiler schedules with fewer live
+// registers (measured on MI355X: grid R=4 FP64 1.47x faster; R <= 2 the same
+// or slightly slower).
+template <int P, int R, int V, int K>
+__device__ inline void tap_step_f64(const double *const (&rows)[2 * R + 1], double (&s)[V])
+{
+#pragma clang fp contract(off)
+    constexpr Tap t = Stencil<P, R>::taps.t[K];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        if constexpr (K == 0)
+            s[v] = rows[t.di + R][v + t.dj] * t.w;
+        else
+            s[v] = s[v] + rows[t.di + R][v + t.dj] * t.w;
+    }
+}
+
+template <int P, int R, int V, size_t... I>
+__device__ inline void tap_sums_f64(const double *const (&rows)[2 * R + 1], double (&s)[V],
+                                    std::index_sequence<I...>)
+{
+    (tap_step_f64<P, R, V, (int)I>(rows, s), ...);
+}
+
 template <int P, int R, size_t... I>
-__device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int v,
-                                     std::index_sequence<I...>)
+__device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int v, std::index_sequence<I...>)
 {
 #pragma clang fp contract(off)
     constexpr const TapList &L = Stencil<P, R>::taps;
@@ -198,22 +223,60 @@ __device__ inline void load_window(const T *row, float (&w)[V + 2 * HP])
     }
 }
 
-// acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs (pair order of
-// make_pairs: by row-offset magnitude).
-template <int P, int R, int V, int HP, size_t... I>
-__device__ inline float pair_sum_f32(const float (&win)[2 * R + 1][V + 2 * HP], int v,
-                                     std::index_sequence<I...>)
+// acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs, in the pair order
+// of make_pairs (by row-offset magnitude |di|, generator order within one).
+// The pairs of one magnitude a read only the rows +a and -a, so the sum walks
+// the magnitudes 0..R with two widened rows live at a time.
+template <int P, int R>
+constexpr int pair_begin(int a)
+{
+    int k = 0;
+    while (k < Stencil<P, R>::pairs.count &&
+           (Stencil<P, R>::pairs.t[k].di < 0 ? -Stencil<P, R>::pairs.t[k].di : Stencil<P, R>::pairs.t[k].di) < a)
+        ++k;
+    return k;
+}
+
+// Pair K for the lane's V columns; wp holds row +|di|, wm row -|di|.
+template <int P, int R, int V, int HP, int K>
+__device__ inline void pair_step(const float (&wp)[V + 2 * HP], const float (&wm)[V + 2 * HP], float (&acc)[V])
 {
 #pragma clang fp contract(off)
-    constexpr const TapList &L = Stencil<P, R>::pairs;
-    float acc = (float)L.t[0].w * (win[R + L.t[0].di][HP + v + L.t[0].dj] -
-                                   win[R - L.t[0].di][HP + v - L.t[0].dj]);
-    ((acc = __builtin_fmaf((float)L.t[I + 1].w,
-                           win[R + L.t[I + 1].di][HP + v + L.t[I + 1].dj] -
-                               win[R - L.t[I + 1].di][HP + v - L.t[I + 1].dj],
-                           acc)),
-     ...);
-    return acc;
+    constexpr Tap t = Stencil<P, R>::pairs.t[K];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const float xp = t.di >= 0 ? wp[HP + v + t.dj] : wm[HP + v + t.dj];
+        const float xm = t.di >= 0 ? wm[HP + v - t.dj] : wp[HP + v - t.dj];
+        if constexpr (K == 0)
+            acc[v] = (float)t.w * (xp - xm);
+        else
+            acc[v] = __builtin_fmaf((float)t.w, xp - xm, acc[v]);
+    }
+}
+
+template <int P, int R, int V, int HP, int B, size_t... I>
+__device__ inline void pair_range(const float (&wp)[V + 2 * HP], const float (&wm)[V + 2 * HP], float (&acc)[V],
+                                  std::index_sequence<I...>)
+{
+    (pair_step<P, R, V, HP, B + (int)I>(wp, wm, acc), ...);
+}
+
+template <typename T, int P, int R, int V, int HP, int A>
+__device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&acc)[V])
+{
+    constexpr int b = pair_begin<P, R>(A), e = pair_begin<P, R>(A + 1);
+    if constexpr (e > b) {
+        float wp[V + 2 * HP];
+        load_window<T, V, HP>(rows[R + A], wp);
+        if constexpr (A == 0) {
+            pair_range<P, R, V, HP, b>(wp, wp, acc, std::make_index_sequence<e - b>{});
+        } else {
+            float wm[V + 2 * HP];
+            load_window<T, V, HP>(rows[R - A], wm);
+            pair_range<P, R, V, HP, b>(wp, wm, acc, std::make_index_sequence<e - b>{});
+        }
+    }
+    if constexpr (A < R) pair_sum_from<T, P, R, V, HP, A + 1>(rows, acc);
 }
 
 // The V outputs of one lane for output row o.  rows[k] points at the LDS row
@@ -227,19 +290,22 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
     typedef typename DTypeTraits<DT>::T T;
     typedef Stencil<P, R> St;
     if constexpr (DT == F64) {
+        double sum[V];
+        if constexpr (R >= 3) {
+            tap_sums_f64<P, R, V>(rows, sum, std::make_index_sequence<St::taps.count>{});
+        } else {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-            res[v] = outv[v] + tap_sum_f64<P, R>(rows, v, std::make_index_sequence<St::taps.count - 1>{});
+            for (int v = 0; v < V; ++v)
+                sum[v] = tap_sum_f64<P, R>(rows, v, std::make_index_sequence<St::taps.count - 1>{});
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) res[v] = outv[v] + sum[v];
     } else {
-        float win[2 * R + 1][V + 2 * HP];
-#pragma unroll
-        for (int k = 0; k < 2 * R + 1; ++k) load_window<T, V, HP>(rows[k], win[k]);
+        float acc[V];
+        pair_sum_from<T, P, R, V, HP, 0>(rows, acc);
         float f[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const float acc = pair_sum_f32<P, R, V, HP>(win, v, std::make_index_sequence<St::pairs.count - 1>{});
-            f[v] = to_f32<T>(outv[v]) + acc;
-        }
+        for (int v = 0; v < V; ++v) f[v] = to_f32<T>(outv[v]) + acc[v];
         from_f32_vec<V>(f, res);
     }
 }

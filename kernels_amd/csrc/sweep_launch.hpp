@@ -45,23 +45,28 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
 }
 
 // The fused, whole-vector sweep is the hot path: it is instantiated for
-// every pipeline depth / cache policy the tuner chooses from.  The other
-// forms use the default.
+// every pipeline depth / cache policy the tuner chooses from (radii beyond
+// the reference's 5: one, two rows in flight with non-temporal stores).  The
+// other forms use the default.
 template <int DT, int P, int R>
 int launch_pr(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s)
 {
     const bool fuse = in_next != nullptr;
     if (fuse && p.mode == MODE_VEC) {
         const int v = p.variant;
-        switch (v) {
-        case 0: return launch_one<DT, P, R, true, MODE_VEC, 1, 0>(p, in, in_next, out, s);
-        case 1: return launch_one<DT, P, R, true, MODE_VEC, 2, 0>(p, in, in_next, out, s);
-        case 2: return launch_one<DT, P, R, true, MODE_VEC, 3, 0>(p, in, in_next, out, s);
-        case 3: return launch_one<DT, P, R, true, MODE_VEC, 1, 1>(p, in, in_next, out, s);
-        case 4: return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
-        case 5: return launch_one<DT, P, R, true, MODE_VEC, 3, 1>(p, in, in_next, out, s);
+        if (v < 0 || v >= kSweepVariants) return set_error(PRK_EINVAL, "unknown sweep variant %d", v);
+        if constexpr (R > kRefMaxRadius) {
+            return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
+        } else {
+            switch (v) {
+            case 0: return launch_one<DT, P, R, true, MODE_VEC, 1, 0>(p, in, in_next, out, s);
+            case 1: return launch_one<DT, P, R, true, MODE_VEC, 2, 0>(p, in, in_next, out, s);
+            case 2: return launch_one<DT, P, R, true, MODE_VEC, 3, 0>(p, in, in_next, out, s);
+            case 3: return launch_one<DT, P, R, true, MODE_VEC, 1, 1>(p, in, in_next, out, s);
+            case 4: return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
+            default: return launch_one<DT, P, R, true, MODE_VEC, 3, 1>(p, in, in_next, out, s);
+            }
         }
-        return set_error(PRK_EINVAL, "unknown sweep variant %d", v);
     }
     if (fuse) {
         if (p.mode == MODE_VEC_LD) return launch_one<DT, P, R, true, MODE_VEC_LD, kPrefetch, 0>(p, in, in_next, out, s);
@@ -82,6 +87,9 @@ int launch_dtype(const SweepPlan &p, const void *in, void *in_next, void *out, h
         case 3: return launch_pr<DT, STAR, 3>(p, in, in_next, out, s);
         case 4: return launch_pr<DT, STAR, 4>(p, in, in_next, out, s);
         case 5: return launch_pr<DT, STAR, 5>(p, in, in_next, out, s);
+        case 6: return launch_pr<DT, STAR, 6>(p, in, in_next, out, s);
+        case 7: return launch_pr<DT, STAR, 7>(p, in, in_next, out, s);
+        case 8: return launch_pr<DT, STAR, 8>(p, in, in_next, out, s);
         }
     } else {
         switch (p.radius) {
@@ -90,6 +98,9 @@ int launch_dtype(const SweepPlan &p, const void *in, void *in_next, void *out, h
         case 3: return launch_pr<DT, GRID, 3>(p, in, in_next, out, s);
         case 4: return launch_pr<DT, GRID, 4>(p, in, in_next, out, s);
         case 5: return launch_pr<DT, GRID, 5>(p, in, in_next, out, s);
+        case 6: return launch_pr<DT, GRID, 6>(p, in, in_next, out, s);
+        case 7: return launch_pr<DT, GRID, 7>(p, in, in_next, out, s);
+        case 8: return launch_pr<DT, GRID, 8>(p, in, in_next, out, s);
         }
     }
     return set_error(PRK_ENOSTENCIL, "no stencil for pattern %d radius %d", p.pattern, p.radius);

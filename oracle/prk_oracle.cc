@@ -48,7 +48,9 @@ constexpr double weight_ab(int pattern, int R, int a, int b)
 
 struct Tap { int di, dj; double w; };
 
-constexpr int max_taps = 11 * 11;
+// radii 1..5: the reference's operators; 6..8: the same formulas extended
+constexpr int max_radius = 8;
+constexpr int max_taps = (2 * max_radius + 1) * (2 * max_radius + 1);
 
 struct TapList {
     int count;
@@ -71,7 +73,7 @@ constexpr TapList make_taps(int pattern, int R)
 
 bool valid_pr(int pattern, int R)
 {
-    return (pattern == PRKO_STAR || pattern == PRKO_GRID) && R >= 1 && R <= 5;
+    return (pattern == PRKO_STAR || pattern == PRKO_GRID) && R >= 1 && R <= max_radius;
 }
 
 // ---------------------------------------------------------------------------
@@ -138,6 +140,9 @@ TiledFn select(int pattern, int R)
         case 3: return Kernel<PRKO_STAR, 3>::tiled;
         case 4: return Kernel<PRKO_STAR, 4>::tiled;
         case 5: return Kernel<PRKO_STAR, 5>::tiled;
+        case 6: return Kernel<PRKO_STAR, 6>::tiled;
+        case 7: return Kernel<PRKO_STAR, 7>::tiled;
+        case 8: return Kernel<PRKO_STAR, 8>::tiled;
         }
     } else if (pattern == PRKO_GRID) {
         switch (R) {
@@ -146,6 +151,9 @@ TiledFn select(int pattern, int R)
         case 3: return Kernel<PRKO_GRID, 3>::tiled;
         case 4: return Kernel<PRKO_GRID, 4>::tiled;
         case 5: return Kernel<PRKO_GRID, 5>::tiled;
+        case 6: return Kernel<PRKO_GRID, 6>::tiled;
+        case 7: return Kernel<PRKO_GRID, 7>::tiled;
+        case 8: return Kernel<PRKO_GRID, 8>::tiled;
         }
     }
     return nullptr;
@@ -351,9 +359,11 @@ int prko_band(int pattern, int radius, int n, int iterations, int init, uint64_t
                                                ? (double)(i + j)
                                                : prko_seeded_value(seed, (uint64_t)(i * n + j));
     const TapList *L = nullptr;
-    static const TapList tables[2][5] = {
-        {make_taps(0, 1), make_taps(0, 2), make_taps(0, 3), make_taps(0, 4), make_taps(0, 5)},
-        {make_taps(1, 1), make_taps(1, 2), make_taps(1, 3), make_taps(1, 4), make_taps(1, 5)}};
+    static const TapList tables[2][max_radius] = {
+        {make_taps(0, 1), make_taps(0, 2), make_taps(0, 3), make_taps(0, 4), make_taps(0, 5), make_taps(0, 6),
+         make_taps(0, 7), make_taps(0, 8)},
+        {make_taps(1, 1), make_taps(1, 2), make_taps(1, 3), make_taps(1, 4), make_taps(1, 5), make_taps(1, 6),
+         make_taps(1, 7), make_taps(1, 8)}};
     L = &tables[pattern][radius - 1];
     const int r0 = std::max(a, radius), r1 = std::min(b, n - radius);
     for (int it = 0; it <= iterations; ++it) {

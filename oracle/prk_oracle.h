@@ -11,7 +11,8 @@
  *   - the term order of Cxx11/generate-cxx-stencil.py:8-39 (bodygen):
  *     column offset outer, row offset inner, zero weights skipped, the
  *     products summed left to right and the sum added to out last
- *   - the operators star1..5 / grid1..5 of Cxx11/stencil_seq.hpp
+ *   - the operators star1..5 / grid1..5 of Cxx11/stencil_seq.hpp, and the
+ *     same formulas and order for radii 6..8 (beyond the reference's table)
  *   - the driver loop, the in+=1 refresh and the L1 norm of
  *     Cxx11/stencil.cc:170-216
  *   - the OpenMP structure of Cxx11/stencil-openmp.cc:182-238 with

@@ -54,9 +54,9 @@ def _rc(fn, *args):
 
 def test_bad_arguments_rejected_before_device():
     L = kernels_amd.lib()
-    # radius outside the generated set: the reference's `nothing`
-    assert L.prk_stencil(0, 6, 100, 32, 16, 16) == -2
-    assert b"radius 6" in L.prk_last_error()
+    # radius outside the table (1..8): the reference's `nothing`
+    assert L.prk_stencil(0, 9, 100, 32, 16, 16) == -2
+    assert b"radius 9" in L.prk_last_error()
     assert L.prk_stencil(0, 0, 100, 32, 16, 16) == -2
     assert L.prk_stencil(2, 2, 100, 32, 16, 16) == -1       # unknown pattern
     assert L.prk_stencil(0, 2, 4, 32, 16, 16) == -1         # 2R+1 > n

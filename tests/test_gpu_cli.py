@@ -30,6 +30,8 @@ def run(args, **env):
     ((10, 1000, 0, "star", 5), {}),                          # tile <= 0 -> n
     ((10, 1000, 32, "star", 2), {"PRK_DTYPE": "f32"}),
     ((4, 777, 32, "grid", 1), {"PRK_DTYPE": "f32"}),         # ragged n
+    ((5, 300, 32, "grid", 8), {}),                           # radii 6..8 extend the reference's table
+    ((5, 300, 32, "star", 7), {"PRK_MODE": "unfused"}),
 ])
 def test_cli_validates(gpu, args, env):
     rc, out = run(args, **env)
@@ -51,7 +53,8 @@ def test_cli_seeded_norm_matches_oracle(gpu, oracle):
 
 
 def test_cli_unsupported_radius_fails(gpu):
-    # the reference selects its no-op operator and then fails validation;
-    # the library refuses the operator and the driver exits with 1
-    rc, out = run((5, 100, 32, "star", 6))
+    # beyond the table (the reference's stops at 5, this one at 8) the
+    # reference selects its no-op operator and then fails validation; the
+    # library refuses the operator and the driver exits with 1
+    rc, out = run((5, 100, 32, "star", 9))
     assert rc == 1 and "ERROR" in out, out

@@ -191,3 +191,34 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
     finally:
         K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
         K.check(L.prk_set_sweep_order(-1, -1)); K.check(L.prk_tuning_clear())
+
+
+EXT = [(p, r) for p in (0, 1) for r in (6, 7, 8)]
+
+
+@pytest.mark.parametrize("pattern,radius", EXT)
+@pytest.mark.parametrize("n", [90, 91])
+def test_extended_radius_bitwise(gpu, oracle, pattern, radius, n):
+    # radii 6..8: the generator's formulas beyond the reference's table,
+    # FP64 and both storage forms bitwise against the oracle
+    iters = 3
+    print(f"ext {NAME[pattern]}{radius} n={n}", flush=True)
+    b0, b1, out = (K.alloc_grid(n, K.F64) for _ in range(3))
+    K.init(K.INIT_SEEDED, 5, K.F64, b0, out)
+    fb, _ = K.run(pattern, radius, K.F64, iters, b0, b1, out)
+    ref_out, ref_in = oracle.run(pattern, radius, n, 32, iters, 1, 5)
+    assert np.array_equal(bits(to_host(out)), bits(ref_out))
+    assert np.array_equal(bits(to_host((b0, b1)[fb])), bits(ref_in))
+    # the drop-in switch with pitch n
+    x = to_dev(ref_in, gpu)
+    o = torch.zeros((n, n), dtype=torch.float64, device=gpu)
+    K.stencil(pattern, n, 32, x, o, radius=radius)
+    oo = np.zeros((n, n))
+    oracle.stencil(pattern, radius, n, 32, ref_in.copy(), oo)
+    assert np.array_equal(bits(to_host(o)), bits(oo))
+    for dtype in (F32, BF16):
+        b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+        K.init(K.INIT_SEEDED, 5, dtype, b0, out)
+        K.run(pattern, radius, dtype, iters, b0, b1, out)
+        lp_out, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 5)
+        assert np.array_equal(to_host(out).view(lp_out.dtype), lp_out), dtype

@@ -25,6 +25,9 @@ from tests._oracle import BF16, F32, REPO, bf16_bits_to_f64, ref_lib
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 OPS = [(p, r) for p in (0, 1) for r in range(1, 6)]
+# radii 6..8 extend the reference's table with the generator's formulas; the
+# checks that need no reference data cover them too
+OPS_ALL = [(p, r) for p in (0, 1) for r in range(1, 9)]
 NAME = {0: "star", 1: "grid"}
 
 
@@ -38,7 +41,16 @@ def test_taps_match_reference_header(oracle, pattern, radius):
         assert w == gw  # the literal round-trips to the same double
 
 
-@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("pattern,radius", OPS_ALL)
+def test_tap_order_is_generator_order(oracle, pattern, radius):
+    # column offset outer, row offset inner, every offset inside the square
+    taps = oracle.taps(pattern, radius)
+    keys = [(dj, di) for di, dj, _ in taps]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+    assert all(max(abs(di), abs(dj)) <= radius for di, dj, _ in taps)
+
+
+@pytest.mark.parametrize("pattern,radius", OPS_ALL)
 def test_weight_identities(oracle, pattern, radius):
     taps = oracle.taps(pattern, radius)
     nnz = 4 * radius if pattern == 0 else 4 * radius * radius + 2 * radius
@@ -85,7 +97,7 @@ def test_oracle_matches_reference_build(oracle, pattern, radius, n, t):
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
 
 
-@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("pattern,radius", OPS_ALL)
 def test_closed_forms(oracle, pattern, radius):
     n = 48
     i, j = np.meshgrid(np.arange(n, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
@@ -123,7 +135,7 @@ def test_tile_size_does_not_change_results(oracle):
         assert np.array_equal(o, outs[0])
 
 
-@pytest.mark.parametrize("pattern,radius", OPS)
+@pytest.mark.parametrize("pattern,radius", OPS_ALL)
 def test_driver_validates_like_reference_ci(oracle, pattern, radius):
     # ci/build-run-prk.sh:340-345: <prog> 10 200 20 {star,grid} {1..5}
     out, fin = oracle.run(pattern, radius, 200, 20, 10, 0)
