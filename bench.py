@@ -296,14 +296,104 @@ def bench_table(a) -> int:
     return 0 if all(c["validated"] is not False for c in cases) else 1
 
 
+TRANSPOSE_WORKLOADS = {"transpose_order32768": 32768}
+
+
+def bench_transpose(a) -> int:
+    """B += A^T, A += 1 on order x order FP64 matrices (Cxx11/transpose.cc):
+    a step is one transpose; W untimed, then K timed; the reference's
+    validation over all W + K transposes."""
+    import torch
+    import kernels_amd as K
+
+    order = TRANSPOSE_WORKLOADS[a.workload]
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    A = torch.empty((order, order), dtype=torch.float64, device="cuda")
+    B = torch.empty_like(A)
+    K.transpose_init(A, B, stream)
+    for _ in range(a.warmup):
+        K.transpose(A, B, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        K.transpose(A, B, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    abserr = K.transpose_check(a.warmup + a.steps - 1, B, stream)
+    del A, B
+    torch.cuda.empty_cache()
+    if a.copy_calibration:
+        copy_calibration(torch, K, stream)
+    bytes_iter = K.transpose_bytes_per_iter(order)
+    gbs = bytes_iter / (ms * 1e-3) / 1e9
+    peak = {} if a.no_peak else measure_hbm_peak(torch, K)
+    res = {
+        "metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (reference init A = 0..order^2-1, B = 0)",
+        "config": {"workload": f"matrix transpose B += A^T, A += 1, order={order}", "order": order},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "algorithmic_bytes_per_launch": bytes_iter},
+        "validated": abserr < 1e-8, "abserr": abserr,
+        "rate_mb_s_reference_formula": round(1e-6 * bytes_iter / (ms * 1e-3), 1),
+    }
+    if peak:
+        res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
+        res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_transpose(order)
+    print(json.dumps(res))
+    return 0 if res["validated"] else 1
+
+
+def cpu_baseline_transpose(order: int):
+    """The reference's OpenMP transpose (Cxx11/transpose-openmp.cc, built
+    from the reference sources into oracle/_ref) on this host's cores, 3
+    timed iterations (plus the untimed first) at the same order."""
+    exe = ROOT / "oracle" / "_ref" / "transpose-openmp"
+    if not exe.exists():
+        return None
+    iters = 3
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([str(exe), str(iters), str(order), "32"], capture_output=True, text=True, env=env,
+                       timeout=900)
+    m = re.search(r"Avg time \(s\):\s*([0-9.eE+-]+)", r.stdout)
+    t = re.search(r"Number of threads\s*=\s*(\d+)", r.stdout)
+    if r.returncode != 0 or not m or "Solution validates" not in r.stdout:
+        return None
+    avg = float(m.group(1))
+    return {"value": 4 * 8 * order * order / avg / 1e9, "unit": "GB/s", "cores": int(t.group(1)) if t else threads,
+            "kind": "reference", "sample": f"{iters} timed + 1 untimed transposes, order={order}",
+            "avg_s_per_iter": avg}
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE, scaled by the copy-kernel calibration (profiles/)."""
     p = ROOT / "profiles" / f"sweep_{workload}.md"
     if not p.exists():
+        p = ROOT / "profiles" / f"{workload}.md"
+    if not p.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
     return float(m.group(1)) if m else None
+
+
+def copy_calibration(torch, K, stream) -> None:
+    """One prk_copy of exactly 2 GiB (2 GiB read, 2 GiB written): the known
+    traffic that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    nb = 2 << 30
+    src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    K.check(K.lib().prk_copy(nb, dst.data_ptr(), src.data_ptr(), stream.cuda_stream))
+    torch.cuda.synchronize()
+    del src, dst
+    torch.cuda.empty_cache()
 
 
 def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
@@ -365,7 +455,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="star2_f64_n30000",
-                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS) + list(TABLE_WORKLOADS))
+                    choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS) + list(TABLE_WORKLOADS)
+                            + list(TRANSPOSE_WORKLOADS))
     ap.add_argument("--serve-rates", type=float, nargs="+", default=list(SERVE_RATES))
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -391,6 +482,8 @@ def main(argv=None) -> int:
         return bench_batch(a)
     if a.workload in TABLE_WORKLOADS:
         return bench_table(a)
+    if a.workload in TRANSPOSE_WORKLOADS:
+        return bench_transpose(a)
     if a.workload in SERVE_WORKLOADS:
         # The first run of this open-loop serving benchmark on MI355X (500
         # req/s for 10 s, 200 GiB pool) ended in a host segmentation fault
@@ -447,13 +540,7 @@ def main(argv=None) -> int:
     del bufs, out
     torch.cuda.empty_cache()
     if a.copy_calibration:
-        nb = 2 << 30
-        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        K.check(K.lib().prk_copy(nb, dst.data_ptr(), src.data_ptr(), stream.cuda_stream))
-        torch.cuda.synchronize()
-        del src, dst
-        torch.cuda.empty_cache()
+        copy_calibration(torch, K, stream)
 
     bytes_iter = K.stencil_bytes_per_iter(n, radius, dtype)
     ms_step = ms / a.steps

@@ -286,6 +286,36 @@ int prk_triad(long long len, double scalar, double *A, const double *B, const do
               void *stream);
 int prk_copy(long long bytes, void *dst, const void *src, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Matrix transpose, the next bandwidth-bound kernel of the suite.  One
+ * iteration is B += A^T and A += 1 on order x order FP64 row-major matrices
+ * (Cxx11/transpose.cc:118-131; the HIP driver's kernels,
+ * Cxx11/transpose-hip.cc:64-119, launched once per iteration at :212-219).
+ * A and B are separate device matrices of pitch `order`.
+ *   prk_transpose        one iteration on the null stream; `tile` is the
+ *                        reference's tiling hint and changes no result
+ *   prk_transpose_ex     one iteration on `stream`
+ *   prk_transpose_init   A = 0, 1, ..., order^2 - 1, B = 0
+ *                        (Cxx11/transpose.cc:110-112)
+ *   prk_transpose_run    iterations + 1 iterations, the last `iterations`
+ *                        timed (the reference's loop, :114-133); blocks
+ *   prk_transpose_check  the reference's validation sum
+ *                        sum |B[j][i] - ((i*order + j)(K+1) + K(K+1)/2)|
+ *                        (:137-146) on the device; blocks; 0 = validates
+ * ------------------------------------------------------------------------ */
+int prk_transpose(int order, int tile, double *A, double *B);
+int prk_transpose_ex(int order, double *A, double *B, void *stream);
+int prk_transpose_init(int order, double *A, double *B, void *stream);
+int prk_transpose_run(int order, int iterations, double *A, double *B, void *stream, double *avg_seconds);
+int prk_transpose_check(int order, int iterations, const double *B, double *abserr, void *stream);
+/* Tuning knob (process-wide, results unchanged): tile shape (rows x
+ * columns of A, threads) 0 = 64x64, 1 = 128x64, 2 = 64x128 (default),
+ * 3 = 32x64, 4 = 128x128 on 1024 threads, 5 = 32x256, 6 = 256x32,
+ * 7 = 64x128 on 512 threads; tile order 0 = row-major (default), 1 =
+ * column-major, 2 = groups of eight tile rows; -1 = default.  Odd orders
+ * always use 64x64 tiles with 8-byte accesses. */
+int prk_set_transpose_variant(int shape, int mode);
+
 #ifdef __cplusplus
 }
 #endif

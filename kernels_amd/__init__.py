@@ -13,5 +13,6 @@ from ._lib import (  # noqa: F401
 )
 from .ops import (  # noqa: F401
     DTYPES, add, alloc_grid, init, l1norm, nominal_flops, run, stencil, stencil_bytes_per_iter,
-    stencil_ex, sweep, torch_dtype, OPERATORS,
+    stencil_ex, sweep, torch_dtype, OPERATORS, transpose, transpose_bytes_per_iter, transpose_check,
+    transpose_init, transpose_run,
 )

@@ -41,8 +41,9 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["prk_api.hip", "sweep_f64.hip", "sweep_f32.hip", "sweep_bf16.hip",
-               "batch.hip", "server.hip", "autotune.hip"]
-CLI_SOURCES = ["prk_cli.cpp"]
+               "batch.hip", "server.hip", "autotune.hip", "transpose.hip"]
+# command-line drivers: executable -> source
+CLIS = {"prk-stencil": "prk_cli.cpp", "prk-transpose": "prk_transpose_cli.cpp"}
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
@@ -70,14 +71,11 @@ def build_lib(jobs: int = 8) -> Path:
     return lib
 
 
-def build_cli() -> Path:
-    exe = PKG / "prk-stencil"
-    srcs = [str(CSRC / s) for s in CLI_SOURCES if (CSRC / s).exists()]
-    if not srcs:
-        return exe
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", "-o", str(exe), *srcs,
-          f"-L{PKG}", "-lprkstencil", f"-Wl,-rpath,$ORIGIN"])
-    return exe
+def build_cli() -> None:
+    for name, src in CLIS.items():
+        if (CSRC / src).exists():
+            _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", "-o", str(PKG / name), str(CSRC / src),
+                  f"-L{PKG}", "-lprkstencil", f"-Wl,-rpath,$ORIGIN"])
 
 
 def build_oracle() -> None:

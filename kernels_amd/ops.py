@@ -114,3 +114,36 @@ def nominal_flops(pattern: int, radius: int, n: int) -> int:
     """The reference's flop count per sweep (Cxx11/stencil.cc:221-222)."""
     size = 4 * radius + 1 if pattern == STAR else (2 * radius + 1) ** 2
     return (2 * size + 1) * (n - 2 * radius) ** 2
+
+
+# ---------------------------------------------------------------------------
+# Matrix transpose (B += A^T, A += 1), order x order FP64 row-major.
+# ---------------------------------------------------------------------------
+def transpose(A: torch.Tensor, B: torch.Tensor, stream=None) -> None:
+    order = A.shape[0]
+    check(lib().prk_transpose_ex(order, _ptr(A), _ptr(B), _stream(stream)))
+
+
+def transpose_init(A: torch.Tensor, B: torch.Tensor, stream=None) -> None:
+    check(lib().prk_transpose_init(A.shape[0], _ptr(A), _ptr(B), _stream(stream)))
+
+
+def transpose_run(iterations: int, A: torch.Tensor, B: torch.Tensor, stream=None) -> float:
+    """iterations + 1 transposes; returns the average seconds of the last
+    `iterations`."""
+    avg = ctypes.c_double(0.0)
+    check(lib().prk_transpose_run(A.shape[0], iterations, _ptr(A), _ptr(B), _stream(stream), ctypes.byref(avg)))
+    return avg.value
+
+
+def transpose_check(iterations: int, B: torch.Tensor, stream=None) -> float:
+    """The reference's validation sum of absolute differences (0 = exact)."""
+    v = ctypes.c_double(0.0)
+    check(lib().prk_transpose_check(B.shape[0], iterations, _ptr(B), ctypes.byref(v), _stream(stream)))
+    return v.value
+
+
+def transpose_bytes_per_iter(order: int) -> int:
+    """Read and write A and B once: the reference's 4 * 8 * order^2."""
+    return 4 * 8 * order * order
+

@@ -491,4 +491,43 @@ int prko_omp_run(int pattern, int radius, int n, int t, int iterations, int thre
     return 0;
 }
 
+int prko_transpose(int order, int tile, double *A, double *B)
+{
+    if (order < 1 || !A || !B || A == B) return -1;
+    if (tile <= 0 || tile >= order) tile = order;
+    for (long it = 0; it < order; it += tile)
+        for (long jt = 0; jt < order; jt += tile)
+            for (long i = it; i < std::min<long>(order, it + tile); i++)
+                for (long j = jt; j < std::min<long>(order, jt + tile); j++) {
+                    B[i * order + j] += A[j * order + i];
+                    A[j * order + i] += 1.0;
+                }
+    return 0;
+}
+
+int prko_transpose_run(int order, int tile, int iterations, double *A, double *B)
+{
+    if (order < 1 || iterations < 0 || !A || !B || A == B) return -1;
+    const long nn = (long)order * order;
+    for (long k = 0; k < nn; ++k) {
+        A[k] = (double)k;
+        B[k] = 0.0;
+    }
+    for (int iter = 0; iter <= iterations; ++iter)
+        if (prko_transpose(order, tile, A, B)) return -1;
+    return 0;
+}
+
+double prko_transpose_abserr(int order, int iterations, const double *B)
+{
+    const double addit = (iterations + 1.) * (iterations / 2.);
+    double abserr = 0.0;
+    for (long j = 0; j < order; j++)
+        for (long i = 0; i < order; i++) {
+            const long ij = i * order + j, ji = j * order + i;
+            abserr += std::fabs(B[ji] - ((double)ij * (1. + iterations) + addit));
+        }
+    return abserr;
+}
+
 }  // extern "C"

@@ -108,6 +108,19 @@ float prko_bf16_to_f32(uint16_t h);
 int prko_omp_run(int pattern, int radius, int n, int t, int iterations,
                  int threads, double *avg_s, double *norm, int *threads_used);
 
+/* Matrix transpose, one iteration of Cxx11/transpose.cc:118-131 with its
+ * tiling (tile <= 0 or >= order: untiled): B[i][j] += A[j][i];
+ * A[j][i] += 1.0, visiting (i, j) tile by tile.  order x order row-major. */
+int prko_transpose(int order, int tile, double *A, double *B);
+
+/* The transpose driver (Cxx11/transpose.cc:110-133): A = 0..order^2-1,
+ * B = 0, then iterations+1 transposes.  Writes the final A and B. */
+int prko_transpose_run(int order, int tile, int iterations, double *A, double *B);
+
+/* The reference's validation sum (Cxx11/transpose.cc:137-146):
+ * sum over (i, j) of |B[j][i] - ((i*order + j)(K+1) + K(K+1)/2)|. */
+double prko_transpose_abserr(int order, int iterations, const double *B);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,10 @@ class Oracle:
         L.prko_bf16_to_f32.restype = ctypes.c_float
         L.prko_bf16_to_f32.argtypes = [ctypes.c_uint16]
         L.prko_omp_run.argtypes = [ctypes.c_int] * 6 + [_dp, _dp, _ip]
+        L.prko_transpose.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.prko_transpose_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.prko_transpose_abserr.restype = ctypes.c_double
+        L.prko_transpose_abserr.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         self.L = L
 
     def taps(self, pattern: int, radius: int):
@@ -115,6 +119,17 @@ class Oracle:
 
     def bf16_to_f32(self, h: int) -> float:
         return self.L.prko_bf16_to_f32(h)
+
+    def transpose(self, order, tile, A, B) -> None:
+        assert self.L.prko_transpose(order, tile, _d(A), _d(B)) == 0
+
+    def transpose_run(self, order, tile, iterations):
+        A = np.empty((order, order)); B = np.empty((order, order))
+        assert self.L.prko_transpose_run(order, tile, iterations, _d(A), _d(B)) == 0
+        return A, B
+
+    def transpose_abserr(self, order, iterations, B) -> float:
+        return self.L.prko_transpose_abserr(order, iterations, _d(np.ascontiguousarray(B)))
 
     def omp_run(self, pattern, radius, n, t, iterations, threads=0):
         avg = ctypes.c_double(); norm = ctypes.c_double(); team = ctypes.c_int()

@@ -93,6 +93,23 @@ def test_cli_argument_errors():
         assert msg in r.stdout, (args, r.stdout)
 
 
+def test_transpose_cli_argument_errors():
+    exe = REPO / "kernels_amd" / "prk-transpose"
+    if not exe.exists():
+        pytest.skip("prk-transpose not built")
+    cases = [
+        ([], "Usage: <# iterations> <matrix order> [variant (0/1/2)]"),
+        (["0", "100"], "ERROR: iterations must be >= 1"),
+        (["10", "0"], "ERROR: Matrix Order must be greater than 0"),
+        (["10", "46341"], "ERROR: matrix dimension too large - overflow risk"),
+        (["10", "100", "3"], "Please select a valid variant (0: naive 1: coalesced, 2: no bank conflicts)"),
+    ]
+    for args, msg in cases:
+        r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, (args, r.stdout)
+        assert msg in r.stdout, (args, r.stdout)
+
+
 def test_tuning_table_roundtrip(tmp_path):
     L = kernels_amd.lib()
     T = kernels_amd.Tuning

@@ -58,3 +58,18 @@ def test_cli_unsupported_radius_fails(gpu):
     # library refuses the operator and the driver exits with 1
     rc, out = run((5, 100, 32, "star", 9))
     assert rc == 1 and "ERROR" in out, out
+
+
+TEXE = REPO / "kernels_amd" / "prk-transpose"
+
+
+@pytest.mark.parametrize("args", [(10, 1000), (5, 1001, 0), (3, 64, 1), (2, 4097)])
+def test_transpose_cli_validates(gpu, args):
+    if not TEXE.exists():
+        pytest.fail("prk-transpose not built (python -m kernels_amd.build)")
+    print("prk-transpose", *args, flush=True)
+    r = subprocess.run([str(TEXE), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    assert "Solution validates" in r.stdout
+    m = re.search(r"Rate \(MB/s\): ([0-9.eE+-]+) Avg time \(s\): ([0-9.eE+-]+)", r.stdout)
+    assert m and float(m.group(1)) > 0, r.stdout

@@ -231,3 +231,42 @@ def test_reference_binaries_validate():
 def test_openmp_baseline_validates(oracle):
     avg, norm, team = oracle.omp_run(0, 2, 400, 32, 3, 2)
     assert norm == 8.0 and team == 2 and avg > 0
+
+
+# ---------------------------------------------------------------------------
+# Matrix transpose (Cxx11/transpose.cc)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("order,tile", [(1, 32), (7, 3), (33, 5), (64, 32), (65, 32), (100, 0), (100, 100)])
+def test_transpose_oracle_is_b_plus_a_transposed(oracle, order, tile):
+    # random non-symmetric A: swapped loop indices or a missed refresh fail
+    rng = np.random.default_rng(order * 100 + tile)
+    A = rng.integers(-1000, 1000, (order, order)).astype(np.float64)
+    B = rng.integers(-1000, 1000, (order, order)).astype(np.float64)
+    A0, B0 = A.copy(), B.copy()
+    oracle.transpose(order, tile, A, B)
+    assert np.array_equal(B, B0 + A0.T)
+    assert np.array_equal(A, A0 + 1.0)
+
+
+def test_transpose_driver_validates(oracle):
+    order, K = 100, 10
+    A, B = oracle.transpose_run(order, 32, K)
+    assert oracle.transpose_abserr(order, K, B) == 0.0
+    # closed form: B[j][i] = (i*order + j)(K+1) + K(K+1)/2, A = iota + K + 1
+    i = np.arange(order, dtype=np.float64)
+    assert np.array_equal(B, (i[None, :] * order + i[:, None]) * (K + 1) + K * (K + 1) / 2)
+    assert np.array_equal(A, np.arange(order * order, dtype=np.float64).reshape(order, order) + K + 1)
+    # the check sees a planted error of 0.5
+    B[3, 5] += 0.5
+    assert oracle.transpose_abserr(order, K, B) == 0.5
+
+
+@pytest.mark.parametrize("exe", ["transpose", "transpose-openmp"])
+def test_reference_transpose_binaries_validate(exe):
+    p = REPO / "oracle" / "_ref" / exe
+    if not p.exists():
+        pytest.skip("oracle/_ref not built (no reference tree here)")
+    for args in (["10", "1000", "32"], ["4", "257", "0"]):
+        r = subprocess.run([str(p), *args], capture_output=True, text=True, timeout=120,
+                           env={"OMP_NUM_THREADS": "4", "PATH": "/usr/bin:/bin"})
+        assert r.returncode == 0 and "Solution validates" in r.stdout, r.stdout
