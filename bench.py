@@ -170,9 +170,19 @@ def bench_batch(a) -> int:
     import kernels_amd as K
     from kernels_amd.batch import Batch, one_by_one
 
+    from kernels_amd.batch import autotune, tuning_get
+
     specs = config4_solves()
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    keys = 0 if a.no_tune else autotune(specs, sample=16, reps=2)
+    t_tune = time.perf_counter() - t0
+    heights = {}
+    for p_, r_, d_, n_, *_ in specs:
+        k = (p_, r_, n_.bit_length() - 1)
+        if k not in heights:
+            heights[k] = tuning_get(p_, r_, d_, n_)
     t0 = time.perf_counter()
     b = Batch(specs)
     t_create = time.perf_counter() - t0
@@ -227,6 +237,9 @@ def bench_batch(a) -> int:
         "speedup_vs_one_by_one_f64": round(solves_s * t64 / len(specs), 2),
         "speedup_vs_one_by_one_f32": round(solves_s * t32 / len(specs), 2),
         "create_s": round(t_create, 2),
+        "tile_heights": {"autotuned_keys": keys, "tune_s": round(t_tune, 2),
+                         "by_pattern_radius_log2n": {f"{'star' if k[0] == 0 else 'grid'}{k[1]}/2^{k[2]}": v
+                                                    for k, v in sorted(heights.items())}},
     }
     print(json.dumps(res))
     return 0 if rel32 < 1e-12 else 1

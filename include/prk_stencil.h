@@ -217,6 +217,19 @@ int prk_batch_wait(prk_batch *b, double *norms);
 int prk_batch_output(const prk_batch *b, int idx, int which, void *host, size_t bytes);
 int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles, int *launches);
 void prk_batch_destroy(prk_batch *b);
+/* Tile heights for batched solves, tuned per (pattern, radius, dtype, size
+ * class floor(log2 n)).  prk_batch_autotune groups `solves` by that key,
+ * takes up to `sample` solves of each group (iterations capped at 10),
+ * times each candidate height (16, 32, 64, 128 rows) over `reps` graph
+ * replays of a batch of that group alone, and records the fastest; it
+ * blocks, and writes the number of keys tuned.  prk_batch_create with
+ * rows_per_tile = 0 then takes each grid's height from the table (32 rows
+ * for keys not tuned).  Heights never change results.
+ * prk_batch_tuning_get reports the height a grid would get. */
+int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps, int *keys_tuned);
+int prk_batch_tuning_get(int pattern, int radius, int dtype, int n, int *rows);
+int prk_batch_tuning_clear(void);
+
 /* Tiles per round the schedules of later prk_batch_create calls aim for
  * (process-wide tuning knob; never changes results). */
 int prk_set_batch_round_tiles(long long tiles);

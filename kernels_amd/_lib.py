@@ -78,6 +78,9 @@ SIGNATURES.update({
     "prk_batch_destroy": (None, [c_void_p]),
     "prk_solve_one_by_one": (c_int, [_SOLVE_P, c_int, c_int, c_double_p, c_void_p]),
     "prk_set_batch_round_tiles": (c_int, [c_ll]),
+    "prk_batch_autotune": (c_int, [_SOLVE_P, c_int, c_int, c_int, c_int_p]),
+    "prk_batch_tuning_get": (c_int, [c_int, c_int, c_int, c_int, c_int_p]),
+    "prk_batch_tuning_clear": (c_int, []),
 })
 
 

@@ -1,4 +1,5 @@
-"""Batched solves through the persistent tile-queue kernel (C ABI:
+"""Batched solves: every grid resident in one device pool, round kernels per
+(storage type, pattern, radius) class replayed from one hipGraph (C ABI:
 prk_batch_* in include/prk_stencil.h)."""
 from __future__ import annotations
 
@@ -71,3 +72,24 @@ def one_by_one(specs, dtype_override: int = -1, stream=None) -> np.ndarray:
     check(lib().prk_solve_one_by_one(solves, len(solves), dtype_override,
                                      norms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), stream))
     return norms
+
+
+def autotune(specs, sample: int = 16, reps: int = 2) -> int:
+    """Tune tile heights per (pattern, radius, dtype, size class) on sample
+    batches drawn from specs; later Batch(..., rows_per_tile=0) uses them.
+    Returns the number of keys tuned."""
+    arr = make_solves(specs)
+    k = ctypes.c_int()
+    check(lib().prk_batch_autotune(arr, len(arr), sample, reps, ctypes.byref(k)))
+    return k.value
+
+
+def tuning_get(pattern: int, radius: int, dtype: int, n: int) -> int:
+    r = ctypes.c_int()
+    check(lib().prk_batch_tuning_get(pattern, radius, dtype, n, ctypes.byref(r)))
+    return r.value
+
+
+def tuning_clear() -> None:
+    check(lib().prk_batch_tuning_clear())
+

@@ -7,7 +7,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "batch.hpp"
@@ -25,6 +29,31 @@ int vec_of(int dt) { return dt == F64 ? 2 : dt == F32 ? 4 : 8; }
 // prologue, so batched tiles are taller than the single-grid row blocks
 // (32 measured best for the 4096-grid mix on MI355X).
 int default_tile_rows(int) { return 32; }
+
+// Tuned tile heights, keyed on (pattern, radius, dtype, size class); the
+// size class of n is floor(log2 n).  Filled by prk_batch_autotune.
+typedef std::tuple<int, int, int, int> BKey;
+std::mutex g_brows_mu;
+std::map<BKey, int> g_brows;
+
+int size_class(int n)
+{
+    int k = 0;
+    while ((2 << k) <= n) ++k;
+    return k;
+}
+
+BKey batch_key(const prk_solve &s) { return BKey(s.pattern, s.radius, s.dtype, size_class(s.n)); }
+
+int tile_rows_for(const prk_solve &s, int rows_per_tile)
+{
+    if (rows_per_tile > 0) return rows_per_tile;
+    std::lock_guard<std::mutex> lock(g_brows_mu);
+    auto it = g_brows.find(batch_key(s));
+    return it != g_brows.end() ? it->second : default_tile_rows(s.radius);
+}
+
+const int kBatchCandRows[] = {16, 32, 64, 128};
 
 // Tiles a round aims for: several waves per CU on all 256 CUs, so a round's
 // tail (its slowest waves) is a small part of it.
@@ -202,7 +231,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
         g.pattern = s.pattern;
         g.radius = s.radius;
         g.iters = s.iterations;
-        g.rows = std::min(s.n, rows_per_tile > 0 ? rows_per_tile : default_tile_rows(s.radius));
+        g.rows = std::min(s.n, tile_rows_for(s, rows_per_tile));
         g.strips = (s.n + 64 * V - 1) / (64 * V);
         g.ntiles = g.strips * ((s.n + g.rows - 1) / g.rows);
         g.init_kind = s.init;
@@ -363,6 +392,105 @@ int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles
 }
 
 void prk_batch_destroy(prk_batch *b) { batch_free(b); }
+
+int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps, int *keys_tuned)
+{
+    if (!solves || count < 1 || sample < 1 || reps < 1) return set_error(PRK_EINVAL, "bad batch autotune arguments");
+    int rc;
+    for (int i = 0; i < count; ++i)
+        if ((rc = validate_solve(solves[i]))) return rc;
+    std::map<BKey, std::vector<prk_solve>> picked;
+    for (int i = 0; i < count; ++i) {
+        std::vector<prk_solve> &g = picked[batch_key(solves[i])];
+        if ((int)g.size() < sample) {
+            prk_solve t = solves[i];
+            t.iterations = std::min(t.iterations, 10);  // the ranking needs a few sweeps, not all
+            g.push_back(t);
+        }
+    }
+    // Each group's sample is repeated until a round of it fills the GPU as
+    // the whole batch's rounds do (twice the round target at the default
+    // height): timed alone, a handful of small grids would leave most CUs
+    // idle and reward short tiles for their parallelism alone.
+    std::map<BKey, std::vector<prk_solve>> groups;
+    for (auto &kv : picked) {
+        std::vector<prk_solve> &g = groups[kv.first];
+        long long tiles = 0;
+        for (size_t k = 0; tiles < 2 * g_round_tiles && g.size() < 4096; ++k) {
+            const prk_solve &t = kv.second[k % kv.second.size()];
+            g.push_back(t);
+            const int V = vec_of(t.dtype), rows = std::min(t.n, default_tile_rows(t.radius));
+            tiles += (long long)((t.n + 64 * V - 1) / (64 * V)) * ((t.n + rows - 1) / rows);
+        }
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "autotune stream");
+    if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+    int tuned = 0;
+    for (auto &kv : groups) {
+        if (rc) break;
+        const std::vector<prk_solve> &sub = kv.second;
+        const int nmin = std::min_element(sub.begin(), sub.end(), [](const prk_solve &a, const prk_solve &b) {
+                             return a.n < b.n;
+                         })->n;
+        int best_rows = 0;
+        float best_ms = 0.0f;
+        for (int rows : kBatchCandRows) {
+            if (rc) break;
+            if (rows > 2 * nmin && rows != kBatchCandRows[0]) continue;
+            prk_batch *b = nullptr;
+            rc = prk_batch_create(sub.data(), (int)sub.size(), rows, 0, &b);
+            if (rc) break;
+            rc = prk_batch_run(b, st);  // warm-up
+            if (!rc) rc = check_hip(hipEventRecord(e0, st), "event");
+            for (int r = 0; r < reps && !rc; ++r) rc = prk_batch_run(b, st);
+            if (!rc) rc = check_hip(hipEventRecord(e1, st), "event");
+            if (!rc) rc = check_hip(hipEventSynchronize(e1), "event sync");
+            float ms = 0.0f;
+            if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time");
+            prk_batch_destroy(b);
+            if (!rc && (best_rows == 0 || ms < best_ms)) {
+                best_rows = rows;
+                best_ms = ms;
+            }
+        }
+        if (!rc && best_rows > 0) {
+            std::lock_guard<std::mutex> lock(g_brows_mu);
+            g_brows[kv.first] = best_rows;
+            ++tuned;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (keys_tuned) *keys_tuned = tuned;
+    return rc;
+}
+
+int prk_batch_tuning_get(int pattern, int radius, int dtype, int n, int *rows)
+{
+    if (!rows) return set_error(PRK_EINVAL, "null rows");
+    prk_solve s{};
+    s.pattern = pattern;
+    s.radius = radius;
+    s.dtype = dtype;
+    s.n = n;
+    s.iterations = 0;
+    s.init = PRK_INIT_ANALYTIC;
+    int rc = validate_solve(s);
+    if (rc) return rc;
+    *rows = std::min(n, tile_rows_for(s, 0));
+    return PRK_OK;
+}
+
+int prk_batch_tuning_clear(void)
+{
+    std::lock_guard<std::mutex> lock(g_brows_mu);
+    g_brows.clear();
+    return PRK_OK;
+}
 
 int prk_set_batch_round_tiles(long long tiles)
 {

@@ -156,3 +156,17 @@ def test_server_config_rejected_before_device():
                 ServerConfig(1 << 30, 1, 1, 0)):
         assert L.prk_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -1
     assert L.prk_server_clock() > 0
+
+
+def test_batch_tuning_table_defaults():
+    L = kernels_amd.lib()
+    import ctypes
+    r = ctypes.c_int()
+    check(L.prk_batch_tuning_clear())
+    check(L.prk_batch_tuning_get(0, 2, 1, 1000, ctypes.byref(r)))
+    assert r.value == 32
+    check(L.prk_batch_tuning_get(1, 4, 0, 20, ctypes.byref(r)))
+    assert r.value == 20                       # never taller than the grid
+    assert L.prk_batch_tuning_get(0, 5, 1, 1000, ctypes.byref(r)) == -2   # radius 5 not batched
+    assert L.prk_batch_autotune(None, 0, 4, 1, None) == -1
+

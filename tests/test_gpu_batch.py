@@ -112,3 +112,25 @@ def test_batch_many_small(gpu, oracle, flags):
     info = b.info()
     assert info["launches"] >= 1 and info["tiles"] > 300
     b.close()
+
+
+def test_batch_autotune_keeps_results(gpu):
+    from kernels_amd.batch import autotune, tuning_clear, tuning_get
+    specs = [((i % 2), 1 + i % 4, 1, 60 + 97 * i, 3 + i % 5, 1, 300 + i) for i in range(24)]
+    tuning_clear()
+    base = Batch(specs)
+    base.run(); ref = base.wait(); outs = [base.output(i) for i in (0, 7, 23)]
+    base.close()
+    try:
+        keys = autotune(specs, sample=4, reps=1)
+        assert keys >= 1
+        heights = {tuning_get(p, r, d, n) for p, r, d, n, *_ in specs}
+        assert heights <= {16, 32, 64, 128} | {n for _, _, _, n, *_ in specs}
+        tuned = Batch(specs)
+        tuned.run(); got = tuned.wait()
+        assert np.array_equal(got, ref)
+        for i, o in zip((0, 7, 23), outs):
+            assert np.array_equal(tuned.output(i).view(np.uint8), o.view(np.uint8))
+        tuned.close()
+    finally:
+        tuning_clear()
