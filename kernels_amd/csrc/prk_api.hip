@@ -451,6 +451,9 @@ int prk_l1norm(int dtype, int n, long long pitch, int radius, const void *out, d
     NormWorkspace *ws = nullptr;
     int rc = norm_workspace(&ws);
     if (rc) return rc;
+    // the workspace is shared per device: one blocking norm at a time
+    static std::mutex norm_mu;
+    std::lock_guard<std::mutex> lock(norm_mu);
     hipStream_t s = (hipStream_t)stream;
     rc = enqueue_l1norm(dtype, n, pitch, radius, out, ws->result, ws->partials, s);
     if (rc) return rc;

@@ -96,6 +96,7 @@ struct Req {
     int final_window = -1;
     double norm = 0;
     bool done = false;
+    int pins = 0;  // fetches in progress (release waits for them)
 };
 
 constexpr int kRoundsPerWindow = 8;
@@ -558,20 +559,39 @@ int prk_server_fetch(prk_server *s, uint64_t id, int which, void *host, size_t b
         auto it = s->kept.find(id);
         if (it == s->kept.end()) return set_error(PRK_ENOTFOUND, "no kept output for request %llu", (unsigned long long)id);
         r = it->second;
+        ++r->pins;  // a concurrent release of this id waits until the copy is done
     }
     const size_t es = dtype_size(r->s.dtype);
-    if (bytes < (size_t)r->g.n * r->g.n * es) return set_error(PRK_EINVAL, "host buffer too small");
-    const void *src = which == 0 ? r->g.out : ((r->s.iterations + 1) % 2 ? r->g.buf1 : r->g.buf0);
-    return check_hip(hipMemcpy2D(host, (size_t)r->g.n * es, src, (size_t)r->g.pitch * es, (size_t)r->g.n * es,
-                                 (size_t)r->g.n, hipMemcpyDeviceToHost),
-                     "server fetch");
+    int rc;
+    if (bytes < (size_t)r->g.n * r->g.n * es) {
+        rc = set_error(PRK_EINVAL, "host buffer too small");
+    } else {
+        const void *src = which == 0 ? r->g.out : ((r->s.iterations + 1) % 2 ? r->g.buf1 : r->g.buf0);
+        rc = check_hip(hipMemcpy2D(host, (size_t)r->g.n * es, src, (size_t)r->g.pitch * es, (size_t)r->g.n * es,
+                                   (size_t)r->g.n, hipMemcpyDeviceToHost),
+                       "server fetch");
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        --r->pins;
+    }
+    s->cv_idle.notify_all();
+    return rc;
 }
 
 int prk_server_release(prk_server *s, uint64_t id)
 {
     if (!s) return set_error(PRK_EINVAL, "null server");
-    std::lock_guard<std::mutex> lock(s->mu);
+    std::unique_lock<std::mutex> lock(s->mu);
     auto it = s->kept.find(id);
+    if (it == s->kept.end()) return set_error(PRK_ENOTFOUND, "no kept output for request %llu", (unsigned long long)id);
+    // fetches of this id finish first (look the id up again on every wake:
+    // another release may have taken it meanwhile)
+    s->cv_idle.wait(lock, [&] {
+        auto j = s->kept.find(id);
+        return j == s->kept.end() || j->second->pins == 0;
+    });
+    it = s->kept.find(id);
     if (it == s->kept.end()) return set_error(PRK_ENOTFOUND, "no kept output for request %llu", (unsigned long long)id);
     s->pool.release(it->second->off, it->second->bytes);
     delete it->second;

@@ -324,6 +324,9 @@ int prk_transpose_check(int order, int iterations, const double *B, double *abse
     double *ws = nullptr;
     int rc = check_workspace(&ws);
     if (rc) return rc;
+    // the workspace is shared per device: one blocking check at a time
+    static std::mutex check_mu;
+    std::lock_guard<std::mutex> lock(check_mu);
     hipLaunchKernelGGL(transpose_check_kernel, dim3(kChkBlocks), dim3(kChkThreads), 0, stream, B, order, iterations,
                        ws);
     hipLaunchKernelGGL(transpose_check_final, dim3(1), dim3(kChkThreads), 0, stream, (const double *)ws,

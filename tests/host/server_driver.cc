@@ -4,7 +4,8 @@
 // was answered once with its analytic norm 2(K+1) (written by the HIP
 // stand-in in mock_hip.cc) and that the pool is empty at the end.  A second
 // phase keeps some outputs, fetches and releases them while the worker runs,
-// under a budget small enough to make requests wait for memory.
+// under a budget small enough to make requests wait for memory.  A third
+// phase fetches kept outputs on one thread while another releases them.
 //
 // usage: server_driver <rate per s> <seconds> <budget GiB> <check every entry 0|1>
 #include <chrono>
@@ -143,6 +144,39 @@ int main(int argc, char **argv)
         prk_server_stats_get(s, &st);
         std::printf("tight budget: fetched %d, pool in use %lld\n", fetched, st.pool_used);
         fails += st.pool_used != 0;
+        prk_server_destroy(s);
+    }
+    // phase 3: one thread fetches kept outputs while another releases them
+    {
+        std::mt19937_64 rng(13);
+        prk_server_config cfg{256ll << 20, 256, 4096, 16384};
+        prk_server *s = nullptr;
+        if (prk_server_create(&cfg, &s)) return 1;
+        std::vector<uint64_t> ids;
+        for (int i = 0; i < 60; ++i) {
+            const prk_solve sv = random_solve(rng, 16, 300);
+            uint64_t id = 0;
+            if (prk_server_submit(s, &sv, 1, &id) == PRK_OK) ids.push_back(id);
+        }
+        fails += prk_server_drain(s, 600) != PRK_OK;
+        int fetched = 0, gone = 0, released = 0;
+        std::thread reader([&] {
+            std::vector<unsigned char> host(1u << 20);
+            for (int round = 0; round < 3; ++round)
+                for (uint64_t id : ids) {
+                    const int rc = prk_server_fetch(s, id, round & 1, host.data(), host.size());
+                    if (rc == PRK_OK) ++fetched;
+                    else if (rc == PRK_ENOTFOUND) ++gone;
+                }
+        });
+        for (uint64_t id : ids) released += prk_server_release(s, id) == PRK_OK;
+        reader.join();
+        prk_server_stats st{};
+        prk_server_stats_get(s, &st);
+        std::printf("fetch/release race: fetched %d, already released %d, released %d of %zu, pool in use %lld\n",
+                    fetched, gone, released, ids.size(), st.pool_used);
+        fails += released != (int)ids.size() || st.pool_used != 0 || fetched + gone != 3 * (int)ids.size();
+        fails += prk_server_release(s, ids.empty() ? 0 : ids[0]) != PRK_ENOTFOUND;
         prk_server_destroy(s);
     }
     std::printf("entries %lld, final norms %lld, failures %d\n", prk::g_items, prk::g_final, fails);
