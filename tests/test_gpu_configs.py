@@ -12,6 +12,9 @@
   matches its oracle bitwise on row bands, and the reduced-precision outputs
   are compared with the FP64 output point by point over the whole grid.
   The errors are printed per case (one JSON line each) and bounded below.
+* The size limits: n = 46340, the largest grid the reference's driver
+  accepts, in FP64 and FP32; n = 2R + 1 (one interior point) for every
+  radius, pattern and storage type.
 """
 import json
 
@@ -134,3 +137,41 @@ def test_n16384_precision(gpu, oracle, pattern, radius, init, seed):
         del out, fin
     del ref_out, ref_fin
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_largest_grid(gpu, oracle, dtype):
+    # n = 46340, the largest grid the reference's driver accepts
+    # (floor(sqrt(INT_MAX)), Cxx11/prk_util.h): 17.2 GB per FP64 array
+    n, iters, radius = 46340, 2, 2
+    print(f"n=46340 star2 dtype={dtype}", flush=True)
+    out, fin = solve(0, radius, dtype, n, iters, 0, 0)
+    norm = K.l1norm(dtype, radius, out)
+    assert abs(norm - 2.0 * (iters + 1)) <= (1e-8 if dtype == F64 else 1e-4)
+    del out, fin
+    torch.cuda.empty_cache()
+    out, fin = solve(1, radius, dtype, n, iters, 1, 77)
+    for a, b in bands(n, radius)[::2]:   # top and bottom bands
+        if dtype == F64:
+            ref = oracle.band(1, radius, n, iters, 1, 77, a, b)
+            assert np.array_equal(host_rows(out, a, b).view(np.uint64), ref.view(np.uint64)), (a, b)
+        else:
+            ref = oracle.band_lp(dtype, 1, radius, n, iters, 1, 77, a, b)
+            assert np.array_equal(host_rows(out, a, b), ref), (a, b)
+    del out, fin
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("radius", range(1, 9))
+@pytest.mark.parametrize("pattern", [0, 1])
+def test_smallest_grid_per_radius(gpu, oracle, pattern, radius):
+    # n = 2R + 1: a single interior point
+    n, iters = 2 * radius + 1, 4
+    for dtype in (F64, F32, BF16):
+        out, fin = solve(pattern, radius, dtype, n, iters, 1, 5)
+        if dtype == F64:
+            ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 5)
+            assert np.array_equal(host_rows(out, 0, n).view(np.uint64), ref.view(np.uint64))
+        else:
+            ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 5)
+            assert np.array_equal(host_rows(out, 0, n), ref)
