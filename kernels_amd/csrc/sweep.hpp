@@ -474,28 +474,30 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
             // Output row o = ri - R: rows o-R .. o+R live in slots
-            // slot+1 .. slot+2R+1 (mod S).  Rows outside the interior are
-            // computed too and their store goes out of range.
+            // slot+1 .. slot+2R+1 (mod S).  The first 2R rows of a tile only
+            // fill the ring (their output rows belong to the tile above), as
+            // do rows past the interior: the whole wave skips their
+            // arithmetic (o is the same in every lane).
             const int o = ri - R;
-            const bool out_ok = o >= olo && o < ohi;
-            const T *rows[S];
+            if (o >= olo && o < ohi) {
+                const T *rows[S];
 #pragma unroll
-            for (int k2 = 0; k2 < S; ++k2) {
-                int s2 = slot + 1 + k2;
-                s2 = s2 >= S ? s2 - S : s2;
-                rows[k2] = &ring[s2][HP + lane * V];
-            }
-            union { u32x4 v; T e[V]; } res;
-            compute_points<DT, P, R, V, HP>(rows, cur.outv.e, res.e);
+                for (int k2 = 0; k2 < S; ++k2) {
+                    int s2 = slot + 1 + k2;
+                    s2 = s2 >= S ? s2 - S : s2;
+                    rows[k2] = &ring[s2][HP + lane * V];
+                }
+                union { u32x4 v; T e[V]; } res;
+                compute_points<DT, P, R, V, HP>(rows, cur.outv.e, res.e);
 #pragma unroll
-            for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : cur.outv.e[v];
-            const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, st_bytes);
-            if constexpr (MODE == MODE_VEC) {
-                bstore16<AUX_STREAM>(rout, out_ok ? off_main : kOOB, res.v);
-            } else {
+                for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : cur.outv.e[v];
+                const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
+                if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else {
 #pragma unroll
-                for (int v = 0; v < V; ++v)
-                    bstore_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB, res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_main + v * ES, res.e[v]);
+                }
             }
             slot = (slot + 1 == S) ? 0 : slot + 1;
         }
