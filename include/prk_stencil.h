@@ -221,8 +221,9 @@ void prk_batch_destroy(prk_batch *b);
  * class floor(log2 n)).  prk_batch_autotune groups `solves` by that key,
  * takes up to `sample` solves of each group (iterations capped at 10),
  * times each candidate height (16, 32, 64, 128 rows) over `reps` graph
- * replays of a batch of that group alone, and records the fastest; it
- * blocks, and writes the number of keys tuned.  prk_batch_create with
+ * replays of a batch of that group alone (repeated until its rounds fill
+ * the GPU), and records the fastest if it beats the default 32 rows by at
+ * least 5 %; it blocks, and writes the number of keys tuned.  prk_batch_create with
  * rows_per_tile = 0 then takes each grid's height from the table (32 rows
  * for keys not tuned).  Heights never change results.
  * prk_batch_tuning_get reports the height a grid would get. */

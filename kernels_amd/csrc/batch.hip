@@ -435,11 +435,22 @@ int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps,
         const int nmin = std::min_element(sub.begin(), sub.end(), [](const prk_solve &a, const prk_solve &b) {
                              return a.n < b.n;
                          })->n;
+        // The default height is timed first and kept unless a candidate is
+        // at least 5 % faster: timed alone, a class runs on the whole GPU,
+        // while in a mixed batch it shares the GPU with the other classes'
+        // branches, and small gains measured alone do not carry over.
+        const int def_rows = default_tile_rows(sub[0].radius);
         int best_rows = 0;
-        float best_ms = 0.0f;
-        for (int rows : kBatchCandRows) {
+        float best_ms = 0.0f, def_ms = 0.0f;
+        int cand[1 + sizeof kBatchCandRows / sizeof kBatchCandRows[0]];
+        int ncand = 0;
+        cand[ncand++] = def_rows;
+        for (int rows : kBatchCandRows)
+            if (rows != def_rows) cand[ncand++] = rows;
+        for (int ci = 0; ci < ncand; ++ci) {
+            const int rows = cand[ci];
             if (rc) break;
-            if (rows > 2 * nmin && rows != kBatchCandRows[0]) continue;
+            if (rows > 2 * nmin && rows != def_rows) continue;
             prk_batch *b = nullptr;
             rc = prk_batch_create(sub.data(), (int)sub.size(), rows, 0, &b);
             if (rc) break;
@@ -451,7 +462,12 @@ int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps,
             float ms = 0.0f;
             if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "event time");
             prk_batch_destroy(b);
-            if (!rc && (best_rows == 0 || ms < best_ms)) {
+            if (rc) break;
+            if (ci == 0) {
+                def_ms = ms;
+                best_rows = rows;
+                best_ms = ms;
+            } else if (ms < best_ms && ms < 0.95f * def_ms) {
                 best_rows = rows;
                 best_ms = ms;
             }
