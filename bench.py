@@ -363,6 +363,112 @@ def bench_transpose(a) -> int:
     return 0 if res["validated"] else 1
 
 
+SLAB_WORKLOADS = {
+    # name: (pattern, radius, dtype, n, slabs): the middle slab of n rows cut
+    # into `slabs` equal slabs, the share of one GPU of a row-strip split
+    "star2_f64_slab8_n30000": (0, 2, 0, 30000, 8),
+}
+
+
+def bench_slab(a) -> int:
+    """One GPU's share of a row-strip split: the fused sweep of one interior
+    slab (prk_sweep_slab) at its real shape, n/P rows plus R halo rows above
+    and below, n columns.  A step is one slab sweep.  The halo exchange with
+    the neighbouring GPUs is not part of the step (one GPU has no
+    neighbours); the halo rows of the new `in` therefore stay as they are, so
+    the run is timed but not validated here -- tests/test_gpu_slab.py checks
+    slabs with a full exchange against the whole grid bit for bit.  The same
+    sweeps over the whole grid are timed after it for comparison."""
+    import torch
+    import kernels_amd as K
+
+    pattern, radius, dtype, n, parts = SLAB_WORKLOADS[a.workload]
+    rows = n // parts
+    row0 = rows * (parts // 2)
+    top, bot = K.slab_halos(n, radius, row0, rows)
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    tuning = K.Tuning()
+    if a.tuning_file and a.no_tune and os.path.exists(a.tuning_file):
+        loaded = K.lib().prk_tuning_load(a.tuning_file.encode())  # entries read, or < 0
+        if loaded < 0:
+            K.check(loaded)
+    if not a.no_tune:
+        K.check(K.lib().prk_autotune(pattern, radius, dtype, n, 2, tuning))
+        if a.tuning_file:
+            K.check(K.lib().prk_tuning_save(a.tuning_file.encode()))
+    else:
+        K.check(K.lib().prk_tuning_get(pattern, radius, dtype, n, tuning))
+    bufs = [K.alloc_slab(n, radius, row0, rows, dtype) for _ in range(2)]
+    out = K.alloc_slab(n, radius, row0, rows, dtype)
+    # the reference's analytic values at the slab's global rows
+    gi = torch.arange(row0 - top, row0 + rows + bot, device="cuda", dtype=torch.float64)
+    gj = torch.arange(n, device="cuda", dtype=torch.float64)
+    bufs[0].copy_(gi[:, None] + gj[None, :])
+    bufs[1].copy_(bufs[0])
+    out.zero_()
+
+    def timed(step):
+        for _ in range(a.warmup):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(a.steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / a.steps
+
+    cur = [0]
+
+    def slab_step():
+        K.sweep_slab(pattern, radius, dtype, n, row0, rows, bufs[cur[0]], bufs[cur[0] ^ 1], out, stream)
+        cur[0] ^= 1
+
+    ms = timed(slab_step)
+    del bufs, out, gi, gj
+    torch.cuda.empty_cache()
+    if a.copy_calibration:
+        copy_calibration(torch, K, stream)
+    ms_whole = None
+    if not a.no_whole:
+        whole = [K.alloc_grid(n, dtype) for _ in range(3)]
+        K.init(K.INIT_ANALYTIC, 0, dtype, whole[0], whole[2], stream)
+        wc = [0]
+
+        def whole_step():
+            K.sweep(pattern, radius, dtype, whole[wc[0]], whole[wc[0] ^ 1], whole[2], stream)
+            wc[0] ^= 1
+
+        ms_whole = timed(whole_step)
+        del whole
+        torch.cuda.empty_cache()
+    bytes_iter = K.slab_bytes_per_iter(n, radius, dtype, row0, rows)
+    gbs = bytes_iter / (ms * 1e-3) / 1e9
+    halo_bytes = (top + bot) * n * {0: 8, 1: 4, 2: 2}[dtype]
+    res = {
+        "metric": METRIC, "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DT_NAME[dtype],
+        "data": "synthetic (reference analytic init at the slab's global rows, out=0)",
+        "config": {"workload": f"{'star' if pattern == 0 else 'grid'} R={radius} {DT_NAME[dtype]} n={n}: "
+                               f"fused sweep of one of {parts} row slabs (rows {row0}..{row0 + rows - 1} "
+                               f"plus {top}+{bot} halo rows), exchange not included",
+                   "n": n, "slabs": parts, "row0": row0, "rows": rows, "halo_rows": [top, bot]},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "algorithmic_bytes_per_launch": bytes_iter},
+        "tuning": {**tuning.as_dict(), "autotuned": not a.no_tune},
+        "validated": None,
+        "whole_grid_ms_per_sweep": None if ms_whole is None else round(ms_whole, 4),
+        "slab_ms_over_whole_ms_per_slab": None if ms_whole is None else round(ms / (ms_whole / parts), 4),
+        "halo_bytes_received_per_sweep": halo_bytes,
+    }
+    print(json.dumps(res))
+    return 0
+
+
 def cpu_baseline_transpose(order: int):
     """The reference's OpenMP transpose (Cxx11/transpose-openmp.cc, built
     from the reference sources into oracle/_ref) on this host's cores, 3
@@ -469,7 +575,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="star2_f64_n30000",
                     choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS) + list(TABLE_WORKLOADS)
-                            + list(TRANSPOSE_WORKLOADS))
+                            + list(TRANSPOSE_WORKLOADS) + list(SLAB_WORKLOADS))
     ap.add_argument("--serve-rates", type=float, nargs="+", default=list(SERVE_RATES))
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -480,6 +586,8 @@ def main(argv=None) -> int:
                     help="skip the autotuner (use the built-in default tiling)")
     ap.add_argument("--tuning-file", default=None,
                     help="save the autotuned table here (or, with --no-tune, load it from here)")
+    ap.add_argument("--no-whole", action="store_true",
+                    help="slab workloads: skip the whole-grid comparison sweeps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the sweeps, run one 2 GiB copy (known bytes) for counter calibration")
     a = ap.parse_args(argv)
@@ -497,6 +605,8 @@ def main(argv=None) -> int:
         return bench_table(a)
     if a.workload in TRANSPOSE_WORKLOADS:
         return bench_transpose(a)
+    if a.workload in SLAB_WORKLOADS:
+        return bench_slab(a)
     if a.workload in SERVE_WORKLOADS:
         # The first run of this open-loop serving benchmark on MI355X (500
         # req/s for 10 s, 200 GiB pool) ended in a host segmentation fault

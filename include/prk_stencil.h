@@ -102,6 +102,24 @@ int prk_add(int n, int t, double *in);
 int prk_sweep(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
               void *in_next, void *out, void *stream);
 
+/* Fused sweep of one slab: the share of one GPU when the n x n grid is cut
+ * into horizontal slabs, the row-strip form of the decomposition of
+ * MPI1/Stencil/stencil.c (its ranks exchange R ghost rows with their
+ * neighbours, :338-370, apply the stencil to their own interior points,
+ * :406-417, and add 1 to their own points only, :419-420).  The slab owns
+ * global rows [row0, row0+rows).  in, in_next and out hold global rows
+ * [row0-top, row0+rows+bot) at `pitch` elements apart, top = min(R, row0)
+ * and bot = min(R, n-row0-rows): the owned rows with up to R halo rows on
+ * each side.  The call does out += S(in) on the owned rows inside the
+ * interior R <= i,j < n-R and in_next = in + 1 on the owned rows; halo rows
+ * of in_next and out are not written (in_next's halo rows are the
+ * neighbours' to send before the next sweep).  Each point's arithmetic is
+ * that of prk_sweep, so slabs whose halo rows are refreshed from their
+ * neighbours every sweep reproduce the whole-grid results bit for bit.
+ * Errors: PRK_EINVAL when the slab leaves [0, n). */
+int prk_sweep_slab(int pattern, int radius, int dtype, int n, int row0, int rows, long long pitch,
+                   const void *in, void *in_next, void *out, void *stream);
+
 /* Unfused sweep in any dtype: out += S(in) only. */
 int prk_stencil_ex(int pattern, int radius, int dtype, int n, long long pitch, const void *in,
                    void *out, void *stream);

@@ -25,6 +25,8 @@ SIGNATURES = {
     "prk_stencil": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "prk_add": (c_int, [c_int, c_int, c_void_p]),
     "prk_sweep": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "prk_sweep_slab": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p,
+                               c_void_p]),
     "prk_stencil_ex": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p]),
     "prk_add_ex": (c_int, [c_int, c_int, c_ll, c_void_p, c_void_p]),
     "prk_init": (c_int, [c_int, c_u64, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p]),

@@ -209,7 +209,8 @@ __device__ inline void sweep_tile(const BGrid &g, int k, int c0, int r0, int r1,
 {
     const void *in = (k & 1) ? g.buf1 : g.buf0;
     void *nx = (k & 1) ? g.buf0 : g.buf1;
-    wave_tile<DT, P, R, true, MODE_VEC, 2, 1, 1>(in, nx, g.out, g.pitch, g.n, c0, r0, r1, 1, ring, lane);
+    wave_tile<DT, P, R, true, MODE_VEC, 2, 1, 1>(in, nx, g.out, g.pitch, g.n, whole_grid(g.n), c0, r0, r1, 1,
+                                                 ring, lane);
 }
 
 // Two waves per SIMD: the largest stencil (grid, R=4) would otherwise take

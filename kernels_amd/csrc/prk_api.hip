@@ -76,6 +76,7 @@ int plan_sweep(int pattern, int radius, int dtype, int n, long long pitch, const
     plan->pattern = pattern;
     plan->radius = radius;
     plan->n = n;
+    plan->span = whole_grid(n);
     plan->pitch = pitch;
     plan->pad_writable = false;
     plan->mode = !aligned ? MODE_ELEM : (n % V == 0 ? MODE_VEC : MODE_VEC_LD);
@@ -415,6 +416,27 @@ int prk_sweep(int pattern, int radius, int dtype, int n, long long pitch, const 
     SweepPlan p;
     int rc = plan_sweep(pattern, radius, dtype, n, pitch, in, in_next, out, 0, &p);
     if (rc) return rc;
+    return launch_sweep(p, in, in_next, out, (hipStream_t)stream);
+}
+
+int prk_sweep_slab(int pattern, int radius, int dtype, int n, int row0, int rows, long long pitch,
+                   const void *in, void *in_next, void *out, void *stream)
+{
+    if (!in_next) return set_error(PRK_EINVAL, "fused sweep needs in_next");
+    SweepPlan p;
+    int rc = plan_sweep(pattern, radius, dtype, n, pitch, in, in_next, out, 0, &p);
+    if (rc) return rc;
+    if (row0 < 0 || rows < 1 || rows > n - row0)
+        return set_error(PRK_EINVAL, "slab of rows [%d, %lld) is not inside the %d rows of the grid", row0,
+                         (long long)row0 + rows, n);
+    // Local row 0 is global row row0 - top.  The first R local rows are halo
+    // rows or frame rows (top < R only when row0 < R), and so are the last R,
+    // so the outputs, global rows [max(row0, R), min(row0+rows, n-R)), are
+    // local rows [R, nrows-R) as for a whole grid (empty when nrows < 2R+1).
+    const int top = std::min(radius, row0), bot = std::min(radius, n - row0 - rows);
+    p.span.nrows = top + rows + bot;
+    p.span.upd_lo = top;
+    p.span.upd_hi = top + rows;
     return launch_sweep(p, in, in_next, out, (hipStream_t)stream);
 }
 

@@ -42,6 +42,19 @@ __host__ __device__ constexpr double weight_ab(int pattern, int R, int a, int b)
     return 0.0;
 }
 
+// Rows of the array a sweep reads and the rows it refreshes.  Outputs are
+// always rows [R, nrows-R) of the array.  A whole grid is rows [0, n), every
+// row refreshed.  A slab (one share of a row-strip decomposition) is its own
+// rows with up to R halo rows on each side: its first and last R rows are
+// halo rows or the grid's frame, so its outputs are rows [R, nrows-R) too,
+// and only its own rows [upd_lo, upd_hi) are refreshed.
+struct RowSpan {
+    int nrows;           // rows in the array
+    int upd_lo, upd_hi;  // rows that get in_next = in + 1
+};
+
+__host__ __device__ inline RowSpan whole_grid(int n) { return RowSpan{n, 0, n}; }
+
 struct Tap {
     int di, dj;  // row offset, column offset
     double w;

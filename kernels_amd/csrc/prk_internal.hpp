@@ -17,7 +17,8 @@ int check_hip(hipError_t e, const char *what);
 
 // Geometry of one sweep launch.
 struct SweepPlan {
-    int dtype, pattern, radius, n;
+    int dtype, pattern, radius, n;  // n: columns (and rows, for a whole grid)
+    RowSpan span;                   // rows read and written (prk_common.hpp)
     long long pitch;
     int rows_per_wave;
     int mode;           // MODE_ELEM / MODE_VEC / MODE_VEC_LD (sweep.hpp)

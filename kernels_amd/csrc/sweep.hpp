@@ -52,11 +52,12 @@ struct SweepArgs {
     void *in_next;  // nullptr when not fused
     void *out;
     long long pitch;  // elements between rows
-    int n;
+    int n;              // columns
+    RowSpan span;
     int rows_per_wave;  // row block height
     int nstrips;        // ceil(n / (64*V))
     int ngroups;        // ceil(nstrips / 4)
-    int nrowblocks;     // ceil(n / rows_per_wave)
+    int nrowblocks;     // ceil(span.nrows / rows_per_wave)
     int ntiles;         // ngroups * nrowblocks
     int chunk;          // tiles per XCD (order 0) / column groups per XCD (order 2)
     int pad_writable;   // columns n..pitch-1 may be overwritten
@@ -370,14 +371,15 @@ __device__ inline void bstore_elem(__amdgpu_buffer_rsrc_t r, unsigned off, T v)
 //   MODE_ELEM   element loads and stores (rows not 16-byte aligned).
 enum { MODE_ELEM = 0, MODE_VEC = 1, MODE_VEC_LD = 2 };
 
-// One wavefront's tile: columns [c0, c0+64V) of rows [r0, r1) of one grid.
-// ring points at this wave's (2R+1) x LR element LDS ring.  LDAUX adds cache
-// policy bits to every HBM load (the batch kernel bypasses L1 with it, since
-// neighbouring tiles written by other CUs are read there).
+// One wavefront's tile: columns [c0, c0+64V) of rows [r0, r1) of an array of
+// sp.nrows rows and n columns.  ring points at this wave's (2R+1) x LR
+// element LDS ring.  LDAUX adds cache policy bits to every HBM load (the
+// batch kernel bypasses L1 with it, since neighbouring tiles written by
+// other CUs are read there).
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX>
-__device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n, int c0,
-                                 int r0, int r1, int pad_writable, typename DTypeTraits<DT>::T *ring_base,
-                                 int lane)
+__device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                 const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                 typename DTypeTraits<DT>::T *ring_base, int lane)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
@@ -388,8 +390,9 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
     constexpr int AUX_OUT_LD = AUX_STREAM | LDAUX;
     (void)SW;
 
-    const int lo = max(r0 - R, 0), hi = min(r1 + R, n);
-    const int olo = max(r0, R), ohi = min(r1, n - R);
+    const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
+    const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
 
     const T *in = (const T *)in_;
     T *in_next = (T *)in_next_;
@@ -453,7 +456,7 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
             if (lane < 2 * NH) *(u32x4 *)&ring[slot][lh] = cur.halo.v;
 
             if constexpr (FUSE) {
-                const bool nx_ok = ri >= r0 && ri < r1;
+                const bool nx_ok = ri >= ulo && ri < uhi;
                 const __amdgpu_buffer_rsrc_t rnx =
                     row_rsrc(in_next + (long long)(nx_ok ? ri : r0) * pitch, st_bytes);
                 union { u32x4 v; T e[V]; } nx;
@@ -523,9 +526,9 @@ void sweep_kernel(SweepArgs a)
     const int strip = group * kWavesPerBlock + wave;
     if (strip >= a.nstrips) return;
     const int r0 = rb * a.rows_per_wave;
-    const int r1 = min(r0 + a.rows_per_wave, a.n);
-    wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, strip * G::SW, r0, r1,
-                                               a.pad_writable, &lds[wave][0][0], lane);
+    const int r1 = min(r0 + a.rows_per_wave, a.span.nrows);
+    wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW, r0,
+                                               r1, a.pad_writable, &lds[wave][0][0], lane);
 }
 
 }  // namespace prk

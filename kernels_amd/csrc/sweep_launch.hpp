@@ -16,10 +16,11 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
     a.out = out;
     a.pitch = p.pitch;
     a.n = p.n;
+    a.span = p.span;
     a.rows_per_wave = p.rows_per_wave;
     a.nstrips = (p.n + G::SW - 1) / G::SW;
     a.ngroups = (a.nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
-    a.nrowblocks = (p.n + p.rows_per_wave - 1) / p.rows_per_wave;
+    a.nrowblocks = (p.span.nrows + p.rows_per_wave - 1) / p.rows_per_wave;
     a.ntiles = a.ngroups * a.nrowblocks;
     a.pad_writable = p.pad_writable ? 1 : 0;
     a.order = p.order;

@@ -72,6 +72,33 @@ def sweep(pattern: int, radius: int, dtype: int, inp: torch.Tensor, in_next: tor
                           _stream(stream)))
 
 
+def slab_halos(n: int, radius: int, row0: int, rows: int) -> tuple[int, int]:
+    """Halo rows above and below a slab owning global rows [row0, row0+rows)."""
+    return min(radius, row0), min(radius, n - row0 - rows)
+
+
+def alloc_slab(n: int, radius: int, row0: int, rows: int, dtype: int = F64, device: str = "cuda") -> torch.Tensor:
+    """Buffer of one slab: its rows plus its halo rows, n columns, the row
+    stride of alloc_grid."""
+    top, bot = slab_halos(n, radius, row0, rows)
+    es = _ESIZE[dtype]
+    per16 = 16 // es
+    pitch = (n + per16 - 1) // per16 * per16
+    return torch.empty((top + rows + bot, pitch), dtype=_TORCH[dtype], device=device)[:, :n]
+
+
+def sweep_slab(pattern: int, radius: int, dtype: int, n: int, row0: int, rows: int, inp: torch.Tensor,
+               in_next: torch.Tensor, out: torch.Tensor, stream=None) -> None:
+    """Fused sweep of the slab owning global rows [row0, row0+rows) of an
+    n x n grid (prk_sweep_slab): buffers hold those rows and their halo rows."""
+    top, bot = slab_halos(n, radius, row0, rows)
+    for t in (inp, in_next, out):
+        if t.shape[0] != top + rows + bot or t.shape[1] != n:
+            raise ValueError(f"slab buffer of shape {tuple(t.shape)}, expected ({top + rows + bot}, {n})")
+    check(lib().prk_sweep_slab(pattern, radius, dtype, n, row0, rows, _pitch(inp), _ptr(inp), _ptr(in_next),
+                               _ptr(out), _stream(stream)))
+
+
 def stencil_ex(pattern: int, radius: int, dtype: int, inp: torch.Tensor, out: torch.Tensor, stream=None) -> None:
     n = inp.shape[0]
     check(lib().prk_stencil_ex(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(out), _stream(stream)))
@@ -108,6 +135,16 @@ def stencil_bytes_per_iter(n: int, radius: int, dtype: int) -> int:
     s = _ESIZE[dtype]
     m = n - 2 * radius
     return 2 * s * n * n + 2 * s * m * m
+
+
+def slab_bytes_per_iter(n: int, radius: int, dtype: int, row0: int, rows: int) -> int:
+    """Algorithmic HBM bytes of one slab sweep: read in on the slab's rows and
+    halo rows, write in_next on its own rows, read and write out on its own
+    rows inside the interior."""
+    s = _ESIZE[dtype]
+    top, bot = slab_halos(n, radius, row0, rows)
+    out_rows = max(0, min(row0 + rows, n - radius) - max(row0, radius))
+    return s * n * (top + rows + bot) + s * n * rows + 2 * s * (n - 2 * radius) * out_rows
 
 
 def nominal_flops(pattern: int, radius: int, n: int) -> int:

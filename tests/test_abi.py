@@ -67,6 +67,14 @@ def test_bad_arguments_rejected_before_device():
     assert L.prk_sweep(0, 2, 0, 100, 99, 16, 32, 48, None) == -1     # pitch < n
     assert L.prk_sweep(0, 2, 0, 100, 100, 16, 16, 48, None) == -1    # in_next aliases in
     assert L.prk_sweep(0, 2, 0, 100, 100, 16, None, 48, None) == -1  # fused needs in_next
+    # slabs: rows [row0, row0+rows) must lie inside the n rows
+    assert L.prk_sweep_slab(0, 2, 0, 100, -1, 10, 100, 16, 32, 48, None) == -1
+    assert L.prk_sweep_slab(0, 2, 0, 100, 0, 0, 100, 16, 32, 48, None) == -1
+    assert L.prk_sweep_slab(0, 2, 0, 100, 95, 6, 100, 16, 32, 48, None) == -1
+    assert b"not inside" in L.prk_last_error()
+    assert L.prk_sweep_slab(0, 2, 0, 100, 2**31 - 5, 10, 100, 16, 32, 48, None) == -1  # no overflow
+    assert L.prk_sweep_slab(0, 9, 0, 100, 0, 10, 100, 16, 32, 48, None) == -2
+    assert L.prk_sweep_slab(0, 2, 0, 100, 0, 10, 100, 16, None, 48, None) == -1
     assert L.prk_run(0, 2, 0, 100, 100, 0, 16, 32, 48, None, None, None) == -1  # iterations < 1
     assert L.prk_init(5, 0, 0, 10, 10, 16, 32, None) == -1
     assert L.prk_set_rows_per_wave(-1) == -1
