@@ -451,8 +451,12 @@ void server_free(prk_server *s)
     if (s->capture) (void)hipStreamDestroy(s->capture);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->pool_base) (void)hipFree(s->pool_base);
+    // A request is in exactly one of these: queued, active, finishing in a
+    // window that was never retired (the worker stopped on an error), kept.
     for (Req *r : s->pending) delete r;
     for (Req *r : s->active) delete r;
+    for (auto &v : s->inflight)
+        for (Req *r : v) delete r;
     for (auto &kv : s->kept) delete kv.second;
     delete s;
 }

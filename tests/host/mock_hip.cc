@@ -60,6 +60,8 @@ static std::mutex g_mu;
 static std::map<const char *, size_t> g_allocs;
 long long g_items = 0, g_final = 0;
 int g_check_all = 1;  // 0: check only each stage's first work entry
+// Failure injection: the instantiation with this ordinal (1-based) fails.
+long long g_instantiations = 0, g_fail_instantiation = 0;
 
 #define MOCK_CHECK(c, ...)                                                            \
     do {                                                                              \
@@ -208,6 +210,10 @@ hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
 }
 hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t)
 {
+    if (++g_instantiations == g_fail_instantiation) {
+        *e = nullptr;
+        return hipErrorInvalidValue;
+    }
     *e = (hipGraphExec_t) new std::vector<Launch>(*(std::vector<Launch> *)g);
     return hipSuccess;
 }

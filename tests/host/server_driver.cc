@@ -5,7 +5,10 @@
 // stand-in in mock_hip.cc) and that the pool is empty at the end.  A second
 // phase keeps some outputs, fetches and releases them while the worker runs,
 // under a budget small enough to make requests wait for memory.  A third
-// phase fetches kept outputs on one thread while another releases them.
+// phase fetches kept outputs on one thread while another releases them.  A
+// fourth makes one window fail to instantiate: the server must report the
+// error, refuse new requests, and free every request it holds when it is
+// destroyed (LeakSanitizer checks that at exit).
 //
 // usage: server_driver <rate per s> <seconds> <budget GiB> <check every entry 0|1>
 #include <chrono>
@@ -22,6 +25,7 @@
 namespace prk {
 extern long long g_items, g_final;
 extern int g_check_all;
+extern long long g_instantiations, g_fail_instantiation;
 }  // namespace prk
 
 static prk_solve random_solve(std::mt19937_64 &rng, int nmin, int nmax)
@@ -178,6 +182,30 @@ int main(int argc, char **argv)
         fails += released != (int)ids.size() || st.pool_used != 0 || fetched + gone != 3 * (int)ids.size();
         fails += prk_server_release(s, ids.empty() ? 0 : ids[0]) != PRK_ENOTFOUND;
         prk_server_destroy(s);
+    }
+    // phase 4: the third window of a server fails to instantiate
+    {
+        std::mt19937_64 rng(17);
+        prk_server_config cfg{1ll << 30, 256, 4096, 16384};
+        prk_server *s = nullptr;
+        if (prk_server_create(&cfg, &s)) return 1;
+        prk::g_fail_instantiation = prk::g_instantiations + 3;
+        int accepted = 0;
+        for (int i = 0; i < 200; ++i) {
+            prk_solve sv = random_solve(rng, 16, 200);
+            sv.iterations = (int)(rng() % 4);  // finishes inside the window it starts in
+            uint64_t id = 0;
+            accepted += prk_server_submit(s, &sv, i % 3 == 0, &id) == PRK_OK;
+            if (i % 20 == 0) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+        const int drc = prk_server_drain(s, 600);
+        prk_solve sv = random_solve(rng, 16, 200);
+        uint64_t id = 0;
+        const int src = prk_server_submit(s, &sv, 0, &id);
+        std::printf("failed window: drain rc %d, submit after failure rc %d, accepted %d\n", drc, src, accepted);
+        fails += drc != PRK_EHIP || src != PRK_EHIP || prk::g_instantiations < 3;
+        prk_server_destroy(s);
+        prk::g_fail_instantiation = 0;
     }
     std::printf("entries %lld, final norms %lld, failures %d\n", prk::g_items, prk::g_final, fails);
     return fails ? 1 : 0;
