@@ -159,7 +159,8 @@ int prk_set_rows_per_wave(int rows);
 int prk_set_sweep_variant(int variant);
 /* order: workgroup-to-tile order (0 column-major runs per XCD, 1 row-major,
  * 2 column ranges per XCD walked row-major, 3 row-major with the column
- * groups padded to a multiple of the 8 XCDs); wgs_per_cu: cap on resident
+ * groups padded to a multiple of the 8 XCDs, 4..7 order 3 with the group ->
+ * XCD map rotated every 2^(order-4) row bands); wgs_per_cu: cap on resident
  * workgroups per CU (0 = none).  -1 for either: the tuned choice. */
 int prk_set_sweep_order(int order, int wgs_per_cu);
 

@@ -92,7 +92,7 @@ int prk_tuning_get(int pattern, int radius, int dtype, int n, prk_tuning *t)
 int prk_tuning_set(int pattern, int radius, int dtype, int n, const prk_tuning *t)
 {
     if (!t || t->variant < 0 || t->variant >= kSweepVariants || t->rows_per_wave < 1 || t->wgs_per_cu < 0 ||
-        t->wgs_per_cu > 16 || t->order < 0 || t->order > 3)
+        t->wgs_per_cu > 16 || t->order < 0 || t->order > 7)
         return set_error(PRK_EINVAL, "bad tuning");
     std::lock_guard<std::mutex> lock(g_mu);
     g_table[Key(pattern, radius, dtype, n)] = *t;

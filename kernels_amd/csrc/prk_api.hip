@@ -365,8 +365,8 @@ int prk_set_sweep_variant(int variant)
 
 int prk_set_sweep_order(int order, int wgs_per_cu)
 {
-    if (order < -1 || order > 3 || wgs_per_cu < -1 || wgs_per_cu > 16)
-        return set_error(PRK_EINVAL, "order must be -1..3 and wgs_per_cu -1..16 (-1: tuned)");
+    if (order < -1 || order > 7 || wgs_per_cu < -1 || wgs_per_cu > 16)
+        return set_error(PRK_EINVAL, "order must be -1..7 and wgs_per_cu -1..16 (-1: tuned)");
     g_order = order;
     g_wgs_per_cu = wgs_per_cu;
     return PRK_OK;

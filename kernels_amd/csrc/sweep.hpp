@@ -59,7 +59,8 @@ struct SweepArgs {
     int ngroups;        // ceil(nstrips / 4)
     int nrowblocks;     // ceil(span.nrows / rows_per_wave)
     int ntiles;         // ngroups * nrowblocks
-    int chunk;          // tiles per XCD (order 0) / column groups per XCD (order 2)
+    int chunk;          // tiles per XCD (order 0) / column groups per XCD (order 2) /
+                        // column groups padded to a multiple of 8 (orders 3..7)
     int pad_writable;   // columns n..pitch-1 may be overwritten
     int order;          // tile order, see tile_of()
 };
@@ -74,9 +75,23 @@ struct SweepArgs {
 //            it re-reads as halo) runs on the same XCD shortly after it;
 //   order 3: row-major with the column-group count padded to a multiple of 8
 //            (chunk), so column group g always runs on XCD g%8: a tile and
-//            the one below it share an L2 and run one row band apart.
+//            the one below it share an L2 and run one row band apart.  When
+//            the group count is not a multiple of 8 some XCDs get one group
+//            more than others in every band;
+//   orders 4..7: order 3 with the group -> XCD map rotated by one group every
+//            2^(order-4) row bands (group g of band rb on XCD
+//            (g + rb / 2^(order-4)) % 8): a tile and the one below share an
+//            L2 except across a rotation, and over eight rotations every XCD
+//            gets every group, which evens out their loads.
 __device__ inline bool tile_of(const SweepArgs &a, int b, int &group, int &rb)
 {
+    if (a.order >= 4) {
+        rb = b / a.chunk;
+        const int shift = (rb >> (a.order - 4)) & 7;
+        group = b - rb * a.chunk - shift;
+        group = group < 0 ? group + a.chunk : group;
+        return rb < a.nrowblocks && group < a.ngroups;
+    }
     if (a.order == 1) {
         if (b >= a.ntiles) return false;
         rb = b / a.ngroups;

@@ -28,7 +28,7 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
     if (p.order == 1) {
         a.chunk = 0;
         blocks = (unsigned)a.ntiles;
-    } else if (p.order == 3) {
+    } else if (p.order >= 3) {
         a.chunk = (a.ngroups + 7) / 8 * 8;
         blocks = (unsigned)a.chunk * (unsigned)a.nrowblocks;
     } else if (p.order == 2) {

@@ -181,7 +181,8 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
     assert best.gbs > 0 and 1 <= best.rows_per_wave <= 32
     ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 9)
     try:
-        for v, rows, order, wgs in [(-1, 0, -1, -1), (0, 1, 0, 0), (3, 5, 2, 1), (5, 64, 1, 4), (2, 300, 1, 2)]:
+        for v, rows, order, wgs in [(-1, 0, -1, -1), (0, 1, 0, 0), (3, 5, 2, 1), (5, 64, 1, 4), (2, 300, 1, 2),
+                                    (4, 8, 3, 3), (4, 8, 4, 3), (5, 4, 5, 2), (4, 3, 6, 3), (1, 7, 7, 4)]:
             K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
             K.check(L.prk_set_sweep_order(order, wgs))
             b0, b1, out = (K.alloc_grid(n, K.F64) for _ in range(3))
