@@ -57,6 +57,8 @@ class Tuning(ctypes.Structure):
 
 SIGNATURES.update({
     "prk_autotune": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
+    "prk_autotune_buffers": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                     ctypes.POINTER(Tuning)]),
     "prk_tuning_get": (c_int, [c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
     "prk_tuning_set": (c_int, [c_int, c_int, c_int, c_int, ctypes.POINTER(Tuning)]),
     "prk_tuning_clear": (c_int, []),

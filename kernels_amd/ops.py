@@ -99,6 +99,19 @@ def sweep_slab(pattern: int, radius: int, dtype: int, n: int, row0: int, rows: i
                                _ptr(out), _stream(stream)))
 
 
+def autotune(pattern: int, radius: int, dtype: int, buf0: torch.Tensor, buf1: torch.Tensor, out: torch.Tensor,
+             reps: int = 2, stream=None):
+    """Tune the sweep of this shape on the caller's three grids (contents
+    overwritten; no device allocation) and record the result for later
+    sweeps.  Returns the chosen Tuning."""
+    from ._lib import Tuning
+    t = Tuning()
+    n = buf0.shape[0]
+    check(lib().prk_autotune_buffers(pattern, radius, dtype, n, _pitch(buf0), _ptr(buf0), _ptr(buf1), _ptr(out),
+                                     reps, _stream(stream), ctypes.byref(t)))
+    return t
+
+
 def stencil_ex(pattern: int, radius: int, dtype: int, inp: torch.Tensor, out: torch.Tensor, stream=None) -> None:
     n = inp.shape[0]
     check(lib().prk_stencil_ex(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(out), _stream(stream)))

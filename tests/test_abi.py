@@ -76,6 +76,11 @@ def test_bad_arguments_rejected_before_device():
     assert L.prk_sweep_slab(0, 9, 0, 100, 0, 10, 100, 16, 32, 48, None) == -2
     assert L.prk_sweep_slab(0, 2, 0, 100, 0, 10, 100, 16, None, 48, None) == -1
     assert L.prk_run(0, 2, 0, 100, 100, 0, 16, 32, 48, None, None, None) == -1  # iterations < 1
+    # tuning on caller grids: three separate grids, pitch >= n, reps >= 1
+    assert L.prk_autotune_buffers(0, 2, 0, 100, 100, 16, 16, 48, 2, None, None) == -1
+    assert L.prk_autotune_buffers(0, 2, 0, 100, 99, 16, 32, 48, 2, None, None) == -1
+    assert L.prk_autotune_buffers(0, 2, 0, 100, 100, 16, 32, 48, 0, None, None) == -1
+    assert L.prk_autotune_buffers(0, 2, 5, 100, 100, 16, 32, 48, 2, None, None) == -1
     assert L.prk_init(5, 0, 0, 10, 10, 16, 32, None) == -1
     assert L.prk_set_rows_per_wave(-1) == -1
     with pytest.raises(PrkError) as e:

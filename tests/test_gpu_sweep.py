@@ -173,6 +173,26 @@ def test_triad_and_copy(gpu):
     assert torch.equal(D[:n], A[:n])
 
 
+def test_autotune_on_caller_grids(gpu, oracle):
+    # prk_autotune_buffers tunes on the caller's grids (overwriting them) and
+    # records the choice; a run on the same grids afterwards is unchanged
+    n, iters, pattern, radius = 257, 2, 0, 4
+    L = K.lib()
+    try:
+        b0, b1, out = (K.alloc_grid(n, K.F64) for _ in range(3))
+        t = K.autotune(pattern, radius, K.F64, b0, b1, out, reps=1)
+        got = K.Tuning()
+        K.check(L.prk_tuning_get(pattern, radius, K.F64, n, got))
+        assert (got.variant, got.rows_per_wave, got.wgs_per_cu, got.order) == \
+            (t.variant, t.rows_per_wave, t.wgs_per_cu, t.order) and t.gbs > 0
+        K.init(K.INIT_SEEDED, 4, K.F64, b0, out)
+        K.run(pattern, radius, K.F64, iters, b0, b1, out)
+        ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 4)
+        assert np.array_equal(bits(to_host(out)), bits(ref))
+    finally:
+        K.check(L.prk_tuning_clear())
+
+
 def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
     n, iters, pattern, radius = 300, 3, 1, 3
     L = K.lib()
