@@ -592,8 +592,9 @@ def main(argv=None) -> int:
                     help="after the sweeps, run one 2 GiB copy (known bytes) for counter calibration")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU halo-exchange form "
-              "is not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): the halo exchange between GPUs is "
+              "not implemented; one GPU's share of a row-strip split is measured by "
+              "--workload star2_f64_slab8_n30000", file=sys.stderr)
         return 2
     if a.steps < 1 or a.warmup < 0:
         print("need --steps >= 1 and --warmup >= 0", file=sys.stderr)
