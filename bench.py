@@ -356,6 +356,7 @@ def bench_transpose(a) -> int:
     if peak:
         res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
         res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+        res["ratio_to_sweep_mix"] = round(gbs / peak["sweep_mix"], 4)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline_transpose(order)
     print(json.dumps(res))
@@ -516,7 +517,10 @@ def copy_calibration(torch, K, stream) -> None:
 
 
 def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
-    """Own copy and triad kernels: the measured HBM peak on this GPU."""
+    """Own copy, triad and sweep-mix kernels: the measured HBM peak on this
+    GPU, and the rate of a plain elementwise kernel with the sweep's mix of
+    two read and two write streams (a comparison, not a bound: the tiled
+    sweep moves its bytes a little faster than it)."""
     s = torch.cuda.current_stream()
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
@@ -526,7 +530,9 @@ def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
             ("copy", lambda: K.check(K.lib().prk_copy(nbytes, dst.data_ptr(), src.data_ptr(), s.cuda_stream)),
              2 * nbytes),
             ("triad", lambda: K.check(K.lib().prk_triad(src.numel(), 3.0, dst.data_ptr(), src.data_ptr(),
-                                                         c.data_ptr(), s.cuda_stream)), 4 * nbytes)):
+                                                         c.data_ptr(), s.cuda_stream)), 4 * nbytes),
+            ("sweep_mix", lambda: K.check(K.lib().prk_stream_mix(src.numel(), src.data_ptr(), dst.data_ptr(),
+                                                                  c.data_ptr(), s.cuda_stream)), 4 * nbytes)):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -702,6 +708,7 @@ def main(argv=None) -> int:
     if peak:
         res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
         res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+        res["ratio_to_sweep_mix"] = round(gbs / peak["sweep_mix"], 4)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pattern, radius, n) if dtype == 0 else None
     print(json.dumps(res))

@@ -326,6 +326,13 @@ int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override,
 int prk_triad(long long len, double scalar, double *A, const double *B, const double *C,
               void *stream);
 int prk_copy(long long bytes, void *dst, const void *src, void *stream);
+/* The fused sweep's traffic without its stencil: in_next = in + 1 and
+ * out += in over len FP64 values (two streams read, two written, 32 bytes
+ * per value), one 16-byte vector per thread.  A plain elementwise kernel
+ * with the sweep's mix of streams, for comparison: on MI355X the tiled
+ * sweep moves its bytes about 4 % faster than this.  No reference
+ * counterpart; a calibration in the manner of the nstream triad above. */
+int prk_stream_mix(long long len, const double *in, double *in_next, double *out, void *stream);
 
 /* ------------------------------------------------------------------------
  * Matrix transpose, the next bandwidth-bound kernel of the suite.  One

@@ -319,6 +319,27 @@ __global__ __launch_bounds__(256) void triad_kernel(long long len, double scalar
     if (k == 0 && (len & 1)) A[len - 1] += B[len - 1] + scalar * C[len - 1];
 }
 
+// The sweep's two read and two write streams, with no stencil between them
+// (in through the cache as the sweep reads it, out and the stores
+// non-temporal: the fastest of the hint combinations measured on MI355X).
+__global__ __launch_bounds__(256) void stream_mix_kernel(long long len, const double *in, double *in_next,
+                                                         double *out)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (k < len / 2) {
+        const d2 x = ((const d2 *)in)[k];
+        d2 o = __builtin_nontemporal_load(&((d2 *)out)[k]);
+        o += x;
+        __builtin_nontemporal_store(x + 1.0, &((d2 *)in_next)[k]);
+        __builtin_nontemporal_store(o, &((d2 *)out)[k]);
+    }
+    if (k == 0 && (len & 1)) {
+        in_next[len - 1] = in[len - 1] + 1.0;
+        out[len - 1] += in[len - 1];
+    }
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(long long nvec, u32x4 *dst, const u32x4 *src)
 {
     const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -528,6 +549,19 @@ int prk_triad(long long len, double scalar, double *A, const double *B, const do
     const unsigned blocks = (unsigned)std::max<long long>(1, (nv + 255) / 256);
     hipLaunchKernelGGL(triad_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, len, scalar, A, B, C);
     return check_hip(hipGetLastError(), "triad launch");
+}
+
+int prk_stream_mix(long long len, const double *in, double *in_next, double *out, void *stream)
+{
+    if (len < 1 || !in || !in_next || !out || in == in_next || in == out || in_next == out)
+        return set_error(PRK_EINVAL, "stream mix needs three separate arrays of len >= 1");
+    if (((uintptr_t)in | (uintptr_t)in_next | (uintptr_t)out) % 16)
+        return set_error(PRK_EINVAL, "stream mix needs 16-byte aligned arrays");
+    const long long nv = len / 2;
+    if ((nv + 255) / 256 >= (1LL << 32)) return set_error(PRK_EINVAL, "stream mix length too large");
+    const unsigned blocks = (unsigned)std::max<long long>(1, (nv + 255) / 256);
+    hipLaunchKernelGGL(stream_mix_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, len, in, in_next, out);
+    return check_hip(hipGetLastError(), "stream mix launch");
 }
 
 int prk_copy(long long bytes, void *dst, const void *src, void *stream)

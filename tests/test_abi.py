@@ -76,6 +76,9 @@ def test_bad_arguments_rejected_before_device():
     assert L.prk_sweep_slab(0, 9, 0, 100, 0, 10, 100, 16, 32, 48, None) == -2
     assert L.prk_sweep_slab(0, 2, 0, 100, 0, 10, 100, 16, None, 48, None) == -1
     assert L.prk_run(0, 2, 0, 100, 100, 0, 16, 32, 48, None, None, None) == -1  # iterations < 1
+    assert L.prk_stream_mix(0, 16, 32, 48, None) == -1       # len < 1
+    assert L.prk_stream_mix(10, 16, 16, 48, None) == -1      # in_next aliases in
+    assert L.prk_stream_mix(10, 16, 40, 48, None) == -1      # misaligned
     # tuning on caller grids: three separate grids, pitch >= n, reps >= 1
     assert L.prk_autotune_buffers(0, 2, 0, 100, 100, 16, 16, 48, 2, None, None) == -1
     assert L.prk_autotune_buffers(0, 2, 0, 100, 99, 16, 32, 48, 2, None, None) == -1

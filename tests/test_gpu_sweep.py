@@ -171,6 +171,14 @@ def test_triad_and_copy(gpu):
     K.check(K.lib().prk_copy(n * 8, D.data_ptr(), A.data_ptr(), None))
     torch.cuda.synchronize()
     assert torch.equal(D[:n], A[:n])
+    # the sweep's stream mix: in_next = in + 1, out += in (odd length: tail element)
+    x = torch.arange(n + 1, dtype=torch.float64, device=gpu)
+    nx = torch.zeros_like(x)
+    out = torch.full_like(x, 5.0)
+    for _ in range(2):
+        K.check(K.lib().prk_stream_mix(n + 1, x.data_ptr(), nx.data_ptr(), out.data_ptr(), None))
+    torch.cuda.synchronize()
+    assert torch.equal(nx, x + 1) and torch.equal(out, 5.0 + 2 * x)
 
 
 def test_autotune_on_caller_grids(gpu, oracle):
