@@ -77,13 +77,14 @@ def slab_halos(n: int, radius: int, row0: int, rows: int) -> tuple[int, int]:
     return min(radius, row0), min(radius, n - row0 - rows)
 
 
-def alloc_slab(n: int, radius: int, row0: int, rows: int, dtype: int = F64, device: str = "cuda") -> torch.Tensor:
-    """Buffer of one slab: its rows plus its halo rows, n columns, the row
-    stride of alloc_grid."""
+def alloc_slab(n: int, radius: int, row0: int, rows: int, dtype: int = F64, device: str = "cuda",
+               pitch: int | None = None) -> torch.Tensor:
+    """Buffer of one slab: its rows plus its halo rows, n columns, row stride
+    `pitch` (default that of alloc_grid)."""
     top, bot = slab_halos(n, radius, row0, rows)
-    es = _ESIZE[dtype]
-    per16 = 16 // es
-    pitch = (n + per16 - 1) // per16 * per16
+    if pitch is None:
+        per16 = 16 // _ESIZE[dtype]
+        pitch = (n + per16 - 1) // per16 * per16
     return torch.empty((top + rows + bot, pitch), dtype=_TORCH[dtype], device=device)[:, :n]
 
 

@@ -30,13 +30,13 @@ def same_bits(a: torch.Tensor, b: torch.Tensor) -> bool:
 class Slabs:
     """Slabs owning rows [bounds[s], bounds[s+1]) of an n x n grid."""
 
-    def __init__(self, n, radius, dtype, bounds, in0, out0):
+    def __init__(self, n, radius, dtype, bounds, in0, out0, pitch=None):
         self.n, self.radius, self.dtype, self.bounds = n, radius, dtype, bounds
         self.parts = []
         for s in range(len(bounds) - 1):
             row0, rows = bounds[s], bounds[s + 1] - bounds[s]
             top, bot = K.slab_halos(n, radius, row0, rows)
-            bufs = [K.alloc_slab(n, radius, row0, rows, dtype) for _ in range(3)]
+            bufs = [K.alloc_slab(n, radius, row0, rows, dtype, pitch=pitch) for _ in range(3)]
             lo = row0 - top
             bufs[0].copy_(in0[lo: row0 + rows + bot])
             bufs[1].fill_(float("nan"))  # halo rows must come from the exchange
@@ -69,10 +69,10 @@ class Slabs:
         return p[key][p["top"]: p["top"] + p["rows"]]
 
 
-def check_against_whole(pattern, radius, dtype, n, bounds, sweeps, init, seed):
-    g_in, g_nx, g_out = (K.alloc_grid(n, dtype) for _ in range(3))
+def check_against_whole(pattern, radius, dtype, n, bounds, sweeps, init, seed, pitch=None):
+    g_in, g_nx, g_out = (K.alloc_grid(n, dtype, pitch=pitch) for _ in range(3))
     K.init(init, seed, dtype, g_in, g_out)
-    slabs = Slabs(n, radius, dtype, bounds, g_in, g_out)
+    slabs = Slabs(n, radius, dtype, bounds, g_in, g_out, pitch=pitch)
     for k in range(sweeps):
         K.sweep(pattern, radius, dtype, g_in, g_nx, g_out)
         g_in, g_nx = g_nx, g_in
@@ -119,6 +119,16 @@ def test_thin_slabs(gpu, radius):
     print(f"thin slabs {bounds} R={radius}", flush=True)
     check_against_whole(1, radius, F64, n, bounds, sweeps=3, init=1, seed=7)
     check_against_whole(0, radius, F64, n, [0, n], sweeps=2, init=1, seed=8)
+
+
+@pytest.mark.parametrize("dtype", [F64, F32, BF16])
+def test_slabs_with_unaligned_rows(gpu, dtype):
+    # pitch = n with n odd: rows are not 16-byte aligned, so the sweep takes
+    # its element-access path, in the slabs as in the whole grid
+    n, radius = 301, 3
+    bounds = uneven_bounds(n, 4, radius, seed=dtype + 40)
+    print(f"unaligned slabs {bounds} dtype={dtype}", flush=True)
+    check_against_whole(1, radius, dtype, n, bounds, sweeps=3, init=1, seed=12, pitch=n)
 
 
 def test_shard_shape_n30000(gpu):
