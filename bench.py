@@ -550,9 +550,10 @@ def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
 def cpu_baseline(pattern: int, radius: int, n: int):
     """The reference's OpenMP driver (Cxx11/stencil-openmp.cc, built from the
     reference sources into oracle/_ref) on this host's cores; if it was not
-    built, the oracle's OpenMP restatement.  Bounded sample: 3 timed sweeps
-    (plus the warm-up sweep) of the same n x n grid."""
-    iters = 3
+    built, the oracle's OpenMP restatement.  Bounded sample: 8 timed sweeps
+    (plus the warm-up sweep) of the same n x n grid, about 1.6 s on 16 host
+    cores at n=30000 (some 26 core-seconds of work)."""
+    iters = 8
     bytes_iter = 2 * 8 * n * n + 2 * 8 * (n - 2 * radius) ** 2
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     exe = ROOT / "oracle" / "_ref" / "stencil-openmp"
