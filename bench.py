@@ -164,6 +164,28 @@ def solve_bytes(spec) -> int:
     return 2 * s * n * n + (iters + 1) * (2 * s * n * n + 2 * s * m * m) + s * m * m
 
 
+def batch_errors_vs_f64(Batch, K, specs, idx, outs) -> dict:
+    """Errors of FP32-storage results against the same solves run in FP64,
+    over the interior: max and mean absolute and relative (relative to
+    max(|y|, 1e-3 mean|y|)) and normwise L2, the worst over the sample."""
+    import numpy as np
+    b64 = Batch([(s[0], s[1], K.F64) + tuple(s[3:]) for s in (specs[i] for i in idx)])
+    b64.run()
+    b64.wait()
+    worst = {"max_abs": 0.0, "mean_abs": 0.0, "max_rel": 0.0, "mean_rel": 0.0, "normwise_l2": 0.0}
+    for j, i in enumerate(idx):
+        radius, n = specs[i][1], specs[i][3]
+        inner = (slice(radius, n - radius),) * 2
+        y = b64.output(j)[inner]
+        d = np.abs(outs[i].astype(np.float64)[inner] - y)
+        rel = d / np.maximum(np.abs(y), 1e-3 * float(np.abs(y).mean()))
+        e = {"max_abs": float(d.max()), "mean_abs": float(d.mean()), "max_rel": float(rel.max()),
+             "mean_rel": float(rel.mean()), "normwise_l2": float(np.linalg.norm(d) / np.linalg.norm(y))}
+        worst = {k: max(worst[k], e[k]) for k in worst}
+    b64.close()
+    return {"solves": len(idx), **{k: float(f"{v:.3e}") for k, v in worst.items()}}
+
+
 def bench_batch(a) -> int:
     import numpy as np
     import torch
@@ -197,8 +219,12 @@ def bench_batch(a) -> int:
     e1.record(stream)
     norms = b.wait()
     ms = e0.elapsed_time(e1)
+    # the FP32 results of a sample of the solves, against the same solves in FP64
+    sample_idx = list(range(0, len(specs), len(specs) // 16))[:16]
+    sample_out = {i: b.output(i) for i in sample_idx}
     b.close()
     torch.cuda.empty_cache()
+    errors = batch_errors_vs_f64(Batch, K, specs, sample_idx, sample_out)
     solves_s = len(specs) * a.steps / (ms * 1e-3)
     total_bytes = sum(solve_bytes(s) for s in specs)
     # unbatched baselines: the same solves one grid per launch
@@ -237,6 +263,7 @@ def bench_batch(a) -> int:
         "speedup_vs_one_by_one_f64": round(solves_s * t64 / len(specs), 2),
         "speedup_vs_one_by_one_f32": round(solves_s * t32 / len(specs), 2),
         "create_s": round(t_create, 2),
+        "fp32_vs_fp64_sample": errors,
         "tile_heights": {"autotuned_keys": keys, "tune_s": round(t_tune, 2),
                          "by_pattern_radius_log2n": {f"{'star' if k[0] == 0 else 'grid'}{k[1]}/2^{k[2]}": v
                                                     for k, v in sorted(heights.items())}},
