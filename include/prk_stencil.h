@@ -203,7 +203,7 @@ int prk_tuning_load(const char *path);
  * Batched solves (no reference counterpart: the reference runs one grid per
  * process).  A solve is the whole driver of Cxx11/stencil.cc:170-216 for one
  * grid: initial values, iterations+1 fused sweeps, L1 norm.  A batch holds
- * many solves of any sizes, patterns, radii (1..4), iteration counts and
+ * many solves of any sizes, patterns, radii (1..8), iteration counts and
  * storage types; prk_batch_create allocates all their grids in one device
  * pool (pitch rounded up to 16 bytes; padding columns are scratch), cuts a
  * sliding-window schedule into rounds (one stage of each active solve per

@@ -168,7 +168,11 @@ int launch_round_pr(int radius, const RoundArgs &a, unsigned blocks, hipStream_t
     case 1: hipLaunchKernelGGL((batch_round_kernel<DT, P, 1>), dim3(blocks), dim3(kThreads), 0, s, a); break;
     case 2: hipLaunchKernelGGL((batch_round_kernel<DT, P, 2>), dim3(blocks), dim3(kThreads), 0, s, a); break;
     case 3: hipLaunchKernelGGL((batch_round_kernel<DT, P, 3>), dim3(blocks), dim3(kThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL((batch_round_kernel<DT, P, 4>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((batch_round_kernel<DT, P, 4>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((batch_round_kernel<DT, P, 5>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((batch_round_kernel<DT, P, 6>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    case 7: hipLaunchKernelGGL((batch_round_kernel<DT, P, 7>), dim3(blocks), dim3(kThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL((batch_round_kernel<DT, P, 8>), dim3(blocks), dim3(kThreads), 0, s, a); break;
     }
     return check_hip(hipGetLastError(), "batch round launch");
 }

@@ -38,7 +38,8 @@
 
 namespace prk {
 
-constexpr int kBatchMaxRadius = 4;
+constexpr int kBatchMaxRadius = kMaxRadius;  // batched and served solves: radii 1..8
+constexpr int kPersistentMaxRadius = 4;      // the (disabled) persistent kernel's stencils
 constexpr int kMaxQueues = 8;
 
 struct DDv {
@@ -218,7 +219,7 @@ __device__ inline void sweep_tile(const BGrid &g, int k, int c0, int r0, int r1,
 template <int DT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(BatchArgs a)
 {
-    typedef SweepGeom<DT, kBatchMaxRadius> GM;
+    typedef SweepGeom<DT, kPersistentMaxRadius> GM;
     typedef typename GM::T T;
     constexpr int SW = GM::SW;
     __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][GM::S * GM::LR];

@@ -159,8 +159,8 @@ def test_batch_persistent_form_is_refused():
     assert L.prk_batch_create(arr, 1, 0, 1, ctypes.byref(h)) == -1
     assert b"disabled" in L.prk_last_error()
     assert L.prk_batch_create(arr, 1, 0, 4, ctypes.byref(h)) == -1
-    bad = (Solve * 1)(Solve(0, 5, 1, 64, 1, 1, 0))
-    assert L.prk_batch_create(bad, 1, 0, 0, ctypes.byref(h)) == -2   # radius 5 not batched
+    bad = (Solve * 1)(Solve(0, 9, 1, 64, 1, 1, 0))
+    assert L.prk_batch_create(bad, 1, 0, 0, ctypes.byref(h)) == -2   # radius 9: no stencil
 
 
 def test_server_config_rejected_before_device():
@@ -183,6 +183,6 @@ def test_batch_tuning_table_defaults():
     assert r.value == 32
     check(L.prk_batch_tuning_get(1, 4, 0, 20, ctypes.byref(r)))
     assert r.value == 20                       # never taller than the grid
-    assert L.prk_batch_tuning_get(0, 5, 1, 1000, ctypes.byref(r)) == -2   # radius 5 not batched
+    assert L.prk_batch_tuning_get(0, 9, 1, 1000, ctypes.byref(r)) == -2   # radius 9: no stencil
     assert L.prk_batch_autotune(None, 0, 4, 1, None) == -1
 

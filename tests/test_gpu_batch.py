@@ -114,6 +114,26 @@ def test_batch_many_small(gpu, oracle, flags):
     b.close()
 
 
+def test_batch_large_radii(gpu, oracle):
+    # radii 5..8 (the reference's R=5 and the generator's 6..8) in every
+    # storage type, batched together with small ones, against the oracle
+    specs = [
+        (0, 5, F64, 40, 3, 1, 61), (1, 5, F32, 23, 2, 1, 62), (1, 6, BF16, 70, 2, 1, 63),
+        (0, 7, F64, 100, 1, 1, 64), (1, 8, F64, 17, 2, 1, 65), (0, 8, F32, 300, 2, 0, 0),
+        (1, 7, F32, 131, 3, 1, 66), (0, 6, BF16, 45, 4, 1, 67), (1, 1, F64, 50, 2, 1, 68),
+    ]
+    b = Batch(specs)
+    b.run()
+    norms = b.wait()
+    for i, s in enumerate(specs):
+        print(f"solve {i}: {s}", flush=True)
+        out, fin, norm = ref_solve(oracle, s)
+        assert np.array_equal(b.output(i, 0).view(np.uint8), out.view(np.uint8)), s
+        assert np.array_equal(b.output(i, 1).view(np.uint8), fin.view(np.uint8)), s
+        assert abs(norms[i] - norm) <= 1e-13 * abs(norm) + 1e-300, (s, norms[i], norm)
+    b.close()
+
+
 def test_batch_autotune_keeps_results(gpu):
     from kernels_amd.batch import autotune, tuning_clear, tuning_get
     specs = [((i % 2), 1 + i % 4, 1, 60 + 97 * i, 3 + i % 5, 1, 300 + i) for i in range(24)]
