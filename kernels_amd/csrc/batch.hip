@@ -80,7 +80,7 @@ struct prk_batch {
     BGrid *d_grids = nullptr;
     RWork *d_work = nullptr;
     double *d_norms = nullptr;
-    // classes: storage type x pattern x radius (1..4)
+    // classes: storage type x pattern x radius (1..8)
     static constexpr int kClasses = 3 * 2 * kBatchMaxRadius;
     std::vector<Round> rounds[kClasses];
     long long total_tiles = 0;

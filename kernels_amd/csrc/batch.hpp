@@ -1,5 +1,5 @@
 // Batched solves: many independent solves (grids of any size, star/grid,
-// radius 1..4, their own iteration counts) in a few launches.
+// radius 1..8, their own iteration counts) in a few launches.
 //
 // Two execution forms share the tile code below.
 //   Round form (batch_round_kernel, the one in use): the host cuts the
