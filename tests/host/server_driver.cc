@@ -28,12 +28,12 @@ extern int g_check_all;
 extern long long g_instantiations, g_fail_instantiation;
 }  // namespace prk
 
-static prk_solve random_solve(std::mt19937_64 &rng, int nmin, int nmax)
+static prk_solve random_solve(std::mt19937_64 &rng, int nmin, int nmax, int rmax = 4)
 {
     std::uniform_real_distribution<double> u(std::log((double)nmin), std::log((double)nmax));
     prk_solve s{};
     s.pattern = (int)(rng() % 2);
-    s.radius = 1 + (int)(rng() % 4);
+    s.radius = 1 + (int)(rng() % rmax);
     s.n = (int)std::lround(std::exp(u(rng)));
     s.iterations = 1 + (int)(rng() % 200);
     const int d = (int)(rng() % 10);
@@ -109,7 +109,7 @@ int main(int argc, char **argv)
         std::vector<unsigned char> host(8u << 20);
         int fetched = 0;
         for (int i = 0; i < 400; ++i) {
-            const prk_solve sv = random_solve(rng, 16, 700);
+            const prk_solve sv = random_solve(rng, 17, 700, 8);
             uint64_t id = 0;
             const int keep = (int)(i % 3 == 0);
             const int rc = prk_server_submit(s, &sv, keep, &id);
