@@ -8,6 +8,7 @@ without a GPU."""
 import os
 import shutil
 import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -33,6 +34,38 @@ def build(tmp: Path, sanitizer: str) -> Path:
     subprocess.run([CLANG, f"-fsanitize={sanitizer}", "-o", str(exe), *objs, "-lpthread"], check=True,
                    capture_output=True, text=True)
     return exe
+
+
+def build_shared(tmp: Path) -> Path:
+    """The server and the HIP stand-in as one shared library (AddressSanitizer)."""
+    objs = []
+    for src in SOURCES[:2]:
+        obj = tmp / (src.name + ".pic.o")
+        subprocess.run([HIPCC, "--offload-host-only", "-O1", "-g", "-std=c++17", "-fPIC", "-fno-omit-frame-pointer",
+                        f"-I{ROOT / 'include'}", f"-I{ROOT / 'kernels_amd/csrc'}",
+                        "-Xarch_host", "-fsanitize=address", "-x", "hip", "-c", str(src), "-o", str(obj)],
+                       check=True, capture_output=True, text=True)
+        objs.append(str(obj))
+    so = tmp / "libprkserver_standin.so"
+    subprocess.run([CLANG, "-shared", "-fsanitize=address", "-o", str(so), *objs, "-lpthread"], check=True,
+                   capture_output=True, text=True)
+    return so
+
+
+def test_serving_client_loop(tmp_path):
+    # bench.py's serving client (the served request stream at 500 req/s, a
+    # 200 GiB budget) through the Python wrapper, against the stand-in
+    so = build_shared(tmp_path)
+    rt = sorted(Path(CLANG).resolve().parents[1].glob("lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    if not rt:
+        pytest.skip("no AddressSanitizer runtime")
+    env = dict(os.environ, PRK_STENCIL_LIB=str(so), LD_PRELOAD=str(rt[-1]),
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
+    r = subprocess.run([sys.executable, str(ROOT / "tests/host/serve_client.py"), "500", "3"], capture_output=True,
+                       text=True, timeout=300, env=env)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr[-4000:]
+    assert "ERROR: AddressSanitizer" not in r.stderr
 
 
 @pytest.mark.parametrize("sanitizer,args", [("address", ["500", "2", "200", "0"]),
