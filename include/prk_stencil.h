@@ -208,7 +208,8 @@ int prk_tuning_load(const char *path);
  * pool (pitch rounded up to 16 bytes; padding columns are scratch), cuts a
  * sliding-window schedule into rounds (one stage of each active solve per
  * round, one launch per round over the tiles of all its solves) and
- * captures the whole run in a hipGraph, one branch per storage type.  Every
+ * captures the whole run in a hipGraph, one branch per (storage type,
+ * pattern, radius) class.  Every
  * solve's arithmetic is that of the single-grid sweep, so each result is
  * the same whatever else is in the batch.  rows_per_tile: tile height
  * (0 = default by radius).  flags: 0, or PRK_BATCH_PERSISTENT for the

@@ -312,7 +312,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
         return rc;
     }
 
-    // capture: fork one branch per storage type, its rounds in order, join
+    // capture: fork one branch per class present, its rounds in order, join
     rc = check_hip(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "batch stream");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "batch event");
     if (!rc) rc = check_hip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch event");
