@@ -207,12 +207,12 @@ int prk_tuning_load(const char *path);
  * storage types; prk_batch_create allocates all their grids in one device
  * pool (pitch rounded up to 16 bytes; padding columns are scratch), cuts a
  * sliding-window schedule into rounds (one stage of each active solve per
- * round, one launch per round over the tiles of all its solves) and
- * captures the whole run in a hipGraph, one branch per (storage type,
- * pattern, radius) class.  Every
- * solve's arithmetic is that of the single-grid sweep, so each result is
- * the same whatever else is in the batch.  rows_per_tile: tile height
- * (0 = default by radius).  flags: 0, or PRK_BATCH_PERSISTENT for the
+ * round, one launch per round and (storage type, pattern, radius) class
+ * over the tiles of its solves) and captures the whole run in a hipGraph,
+ * one branch per class.  Every solve's arithmetic is that of the
+ * single-grid sweep, so each result is the same whatever else is in the
+ * batch.  rows_per_tile: tile height (0 = the height prk_batch_autotune
+ * chose for the solve's key, else 32).  flags: 0, or PRK_BATCH_PERSISTENT for the
  * single-launch tile-queue form, which is disabled (it hung on MI355X) and
  * returns PRK_EINVAL.
  *   prk_batch_run     enqueue one run of every solve on `stream`
