@@ -75,18 +75,25 @@ def test_transpose_cli_validates(gpu, args):
     assert m and float(m.group(1)) > 0, r.stdout
 
 
-# The reference CI's stencil invocations, literally (ci/build-run-prk.sh:151,
-# 154-158): "stencil 10 1000" and "stencil 10 200 {star,grid} {1..5}".  The
-# second form passes the pattern where the tile goes, so the driver parses
-# tile = atoi("star") = 0 -> n, pattern "1".."5" -> star, radius 2 -- exactly
-# as the reference's own argument handling does; every run must validate.
-CI_CASES = [("10", "1000")] + [("10", "200", s, r) for s in ("star", "grid") for r in "12345"]
+# The reference CI's stencil invocations, literally.  The Cxx11 drivers
+# (ci/build-run-prk.sh:334,340-345 stencil-vector; :372,375-380
+# stencil-openmp) run "10 1000" and "10 200 20 {star,grid} {1..5}".  The C11
+# section (:151,154-158) runs "10 200 {star,grid} {1..5}", which passes the
+# pattern where the tile goes: the driver parses tile = atoi("star") = 0 ->
+# n, pattern "1".."5" -> star, radius 2 -- exactly as the reference's own
+# argument handling does.  Every run must validate.
+CI_CASES = ([("10", "1000")] + [("10", "200", "20", s, r) for s in ("star", "grid") for r in "12345"]
+            + [("10", "200", s, r) for s in ("star", "grid") for r in "12345"])
 
 
 @pytest.mark.parametrize("args", CI_CASES, ids=["-".join(a) for a in CI_CASES])
 def test_reference_ci_invocations(gpu, args):
     rc, out = run(args)
     assert rc == 0 and "Solution validates" in out, out
+    if len(args) == 5:
+        kind = "star" if args[3] == "star" else "grid"
+        assert "Tile size            = 20" in out and f"Type of stencil      = {kind}" in out
+        assert f"Radius of stencil    = {args[4]}" in out
     if len(args) == 4:
         assert "Tile size            = 200" in out and "Type of stencil      = star" in out
         assert "Radius of stencil    = 2" in out
