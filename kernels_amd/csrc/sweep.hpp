@@ -188,31 +188,10 @@ __device__ inline void plus_one(const typename DTypeTraits<DT>::T *x, typename D
 // Strict-order FP64 sums of the taps for the lane's V output columns (folds
 // over the compile-time tap list, so every index and weight is an
 // immediate).  Each chain keeps the generator's order.  Up to R = 2 the
-// chains run one column after the other; from R = 3 on the V chains advance
-// together, tap by tap, which the compiler schedules with fewer live
-// registers (measured on MI355X: grid R=4 FP64 1.47x faster; R <= 2 the same
-// or slightly slower).
-template <int P, int R, int V, int K>
-__device__ inline void tap_step_f64(const double *const (&rows)[2 * R + 1], double (&s)[V])
-{
-#pragma clang fp contract(off)
-    constexpr Tap t = Stencil<P, R>::taps.t[K];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-        if constexpr (K == 0)
-            s[v] = rows[t.di + R][v + t.dj] * t.w;
-        else
-            s[v] = s[v] + rows[t.di + R][v + t.dj] * t.w;
-    }
-}
-
-template <int P, int R, int V, size_t... I>
-__device__ inline void tap_sums_f64(const double *const (&rows)[2 * R + 1], double (&s)[V],
-                                    std::index_sequence<I...>)
-{
-    (tap_step_f64<P, R, V, (int)I>(rows, s), ...);
-}
-
+// chains run one column after the other; from R = 3 on the two chains
+// advance together, fed column pair by column pair (pair_sums_f64 below),
+// which the compiler schedules with fewer live registers than one chain
+// after the other (measured on MI355X: grid R=4 FP64 1.47x faster).
 template <int P, int R, size_t... I>
 __device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int v, std::index_sequence<I...>)
 {
@@ -221,6 +200,76 @@ __device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int
     double s = rows[L.t[0].di + R][v + L.t[0].dj] * L.t[0].w;
     ((s = s + rows[L.t[I + 1].di + R][v + L.t[I + 1].dj] * L.t[I + 1].w), ...);
     return s;
+}
+
+// The same two chains (V = 2) fed from 16-byte-aligned column pairs.  For an
+// odd R the columns a lane needs start at an odd offset, and reading them as
+// unaligned 8-byte pairs (ds_read2_b64 at offsets 8 and 16 from each lane's
+// 16-byte slot) conflicted in the LDS banks for about 45 % of the kernel's
+// time (grid R=3 and R=5, rocprof SQ_LDS_BANK_CONFLICT).  Here the lane reads
+// whole aligned pairs (c, c+1), c even, with ds_read_b128, and each pair is
+// used where the chains need it: chain 1 takes its taps of column offsets c-1
+// and c (columns c and c+1), chain 0 those of c and c+1.  The pairs go left to
+// right, so each chain still adds its taps in the generator's order.
+template <int P, int R>
+constexpr int tap_begin(int dj)  // first tap (generator order) with column offset >= dj
+{
+    int k = 0;
+    while (k < Stencil<P, R>::taps.count && Stencil<P, R>::taps.t[k].dj < dj) ++k;
+    return k;
+}
+
+template <int P, int R, int C>
+constexpr bool pair_needs_row(int di)  // some tap of columns C-1..C+1 reads row offset di
+{
+    for (int k = tap_begin<P, R>(C - 1); k < tap_begin<P, R>(C + 2); ++k)
+        if (Stencil<P, R>::taps.t[k].di == di) return true;
+    return false;
+}
+
+template <int P, int R, int C, int B>
+__device__ inline void load_pair_rows(const double *const (&rows)[2 * R + 1], double (&x)[2 * R + 1][2])
+{
+    if constexpr (pair_needs_row<P, R, C>(B - R)) {
+        union { u32x4 v; double e[2]; } u;
+        u.v = *(const u32x4 *)(rows[B] + C);
+        x[B][0] = u.e[0];
+        x[B][1] = u.e[1];
+    }
+    if constexpr (B < 2 * R) load_pair_rows<P, R, C, B + 1>(rows, x);
+}
+
+// Tap K of chain CH (output column CH), whose column CH + dj lies in the pair at C.
+template <int P, int R, int CH, int C, int K>
+__device__ inline void pair_tap_f64(const double (&x)[2 * R + 1][2], double &s)
+{
+#pragma clang fp contract(off)
+    constexpr Tap t = Stencil<P, R>::taps.t[K];
+    constexpr int e = CH + t.dj - C;
+    static_assert(e == 0 || e == 1, "tap outside the column pair");
+    if constexpr (K == 0)
+        s = x[t.di + R][e] * t.w;
+    else
+        s = s + x[t.di + R][e] * t.w;
+}
+
+template <int P, int R, int CH, int C, int B, size_t... I>
+__device__ inline void pair_taps_f64(const double (&x)[2 * R + 1][2], double &s, std::index_sequence<I...>)
+{
+    (pair_tap_f64<P, R, CH, C, B + (int)I>(x, s), ...);
+}
+
+template <int P, int R, int C>
+__device__ inline void pair_sums_f64(const double *const (&rows)[2 * R + 1], double (&s)[2])
+{
+    constexpr int c_hi = (R & 1) ? R + 1 : R;  // last pair: chain 1's column R + 1
+    double x[2 * R + 1][2];
+    load_pair_rows<P, R, C, 0>(rows, x);
+    constexpr int b1 = tap_begin<P, R>(C - 1), e1 = tap_begin<P, R>(C + 1);
+    constexpr int b0 = tap_begin<P, R>(C), e0 = tap_begin<P, R>(C + 2);
+    pair_taps_f64<P, R, 1, C, b1>(x, s[1], std::make_index_sequence<e1 - b1>{});
+    pair_taps_f64<P, R, 0, C, b0>(x, s[0], std::make_index_sequence<e0 - b0>{});
+    if constexpr (C + 2 <= c_hi) pair_sums_f64<P, R, C + 2>(rows, s);
 }
 
 // Reduced-precision forms.  Each row a lane reads is fetched from LDS as whole
@@ -308,7 +357,8 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
     if constexpr (DT == F64) {
         double sum[V];
         if constexpr (R >= 3) {
-            tap_sums_f64<P, R, V>(rows, sum, std::make_index_sequence<St::taps.count>{});
+            static_assert(V == 2, "FP64 column pairs");
+            pair_sums_f64<P, R, (R & 1) ? -R - 1 : -R>(rows, sum);
         } else {
 #pragma unroll
             for (int v = 0; v < V; ++v)
