@@ -37,6 +37,9 @@ WORKLOADS = {
     "grid2_f64_n30000": (1, 2, 0, 30000),
     "star2_f32_n30000": (0, 2, 1, 30000),
     "star2_bf16_n30000": (0, 2, 2, 30000),
+    # FP64 grid stencils bound by their tap chains (radius/precision sweep sizes)
+    "grid3_f64_n16384": (1, 3, 0, 16384),
+    "grid5_f64_n16384": (1, 5, 0, 16384),
 }
 DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 BATCH_WORKLOADS = ("batch4096_f32",)
