@@ -187,11 +187,12 @@ __device__ inline void plus_one(const typename DTypeTraits<DT>::T *x, typename D
 
 // Strict-order FP64 sums of the taps for the lane's V output columns (folds
 // over the compile-time tap list, so every index and weight is an
-// immediate).  Each chain keeps the generator's order.  Up to R = 2 the
-// chains run one column after the other; from R = 3 on the two chains
-// advance together, fed column pair by column pair (pair_sums_f64 below),
-// which the compiler schedules with fewer live registers than one chain
-// after the other (measured on MI355X: grid R=4 FP64 1.47x faster).
+// immediate).  Each chain keeps the generator's order.  At R = 1 the chains
+// run one column after the other; from R = 2 on the two chains advance
+// together, fed column pair by column pair (pair_sums_f64 below), which the
+// compiler schedules with fewer live registers than one chain after the
+// other (measured on MI355X: grid R=4 FP64 1.47x faster, star R=2 at
+// n=30000 1.2 % faster; R = 1 0.3 % slower, so it keeps the single chains).
 template <int P, int R, size_t... I>
 __device__ inline double tap_sum_f64(const double *const (&rows)[2 * R + 1], int v, std::index_sequence<I...>)
 {
@@ -356,7 +357,7 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
     typedef Stencil<P, R> St;
     if constexpr (DT == F64) {
         double sum[V];
-        if constexpr (R >= 3) {
+        if constexpr (R >= 2) {
             static_assert(V == 2, "FP64 column pairs");
             pair_sums_f64<P, R, (R & 1) ? -R - 1 : -R>(rows, sum);
         } else {
