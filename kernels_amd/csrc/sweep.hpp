@@ -270,6 +270,13 @@ __device__ inline void pair_sums_f64(const double *const (&rows)[2 * R + 1], dou
     constexpr int b0 = tap_begin<P, R>(C), e0 = tap_begin<P, R>(C + 2);
     pair_taps_f64<P, R, 1, C, b1>(x, s[1], std::make_index_sequence<e1 - b1>{});
     pair_taps_f64<P, R, 0, C, b0>(x, s[0], std::make_index_sequence<e0 - b0>{});
+    // Grid stencils from R = 3: a scheduling fence after each pair keeps the
+    // compiler from hoisting every pair's loads to the top of the row (it
+    // held 212 VGPRs at R=4 and 2 waves per SIMD; with the fence 98 and the
+    // LDS-bound 4; measured on MI355X: grid R=3 +5 %, R=4 +10 %, R=5 +11 %,
+    // R=6..8 +42-48 %).  The star stencils read full columns only in the
+    // centre pair, and lost 1-5 % with it.
+    if constexpr (P == GRID && R >= 3) __builtin_amdgcn_sched_barrier(0);
     if constexpr (C + 2 <= c_hi) pair_sums_f64<P, R, C + 2>(rows, s);
 }
 
