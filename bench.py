@@ -694,6 +694,19 @@ def main(argv=None) -> int:
     t_wall = time.perf_counter() - t_wall
     ms = e0.elapsed_time(e1)
     sweeps = a.warmup + a.steps
+    # Run-to-run spread after the timed region: five more timed runs of up
+    # to 20 sweeps each (they count towards the validated sweep total).
+    rep_n = min(a.steps, 20)
+    rep_ms = []
+    for _ in range(5):
+        e0.record(stream)
+        for _ in range(rep_n):
+            K.sweep(pattern, radius, dtype, bufs[cur], bufs[cur ^ 1], out, stream)
+            cur ^= 1
+        e1.record(stream)
+        e1.synchronize()
+        rep_ms.append(e0.elapsed_time(e1) / rep_n)
+    sweeps += 5 * rep_n
     norm = K.l1norm(dtype, radius, out, stream)
     eps = {0: 1e-8, 1: 1e-4, 2: 1e-2}[dtype]
     # BF16 (8-bit significand) cannot hold i + j + k beyond 256, so the
@@ -735,6 +748,9 @@ def main(argv=None) -> int:
         "mflops_reference_formula": round(K.nominal_flops(pattern, radius, n) / (ms_step * 1e-3) / 1e6, 1),
         "points_per_s": (n - 2 * radius) ** 2 / (ms_step * 1e-3),
         "wall_s_timed_region": round(t_wall, 4),
+        "repeats": {"runs": 5, "sweeps_each": rep_n,
+                    "gbs": [round(bytes_iter / (m * 1e-3) / 1e9, 1) for m in rep_ms],
+                    "gbs_median": round(bytes_iter / (sorted(rep_ms)[2] * 1e-3) / 1e9, 1)},
     }
     if peak:
         res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}

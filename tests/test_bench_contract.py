@@ -45,6 +45,9 @@ def test_default_workload_line(gpu):
     assert roof["bound"] == "hbm" and roof["peak"] == 8000.0 and 0 < roof["frac"] < 1
     assert roof["algorithmic_bytes_per_launch"] == 2 * 8 * 30000 ** 2 + 2 * 8 * 29996 ** 2
     assert set(d["hbm_measured"]) == {"copy", "triad", "sweep_mix"}
+    rep = d["repeats"]
+    assert rep["runs"] == 5 and rep["sweeps_each"] == 3 and len(rep["gbs"]) == 5
+    assert sorted(rep["gbs"])[2] == rep["gbs_median"] and rep["gbs_median"] > 0
 
 
 @pytest.mark.gpu
