@@ -385,6 +385,129 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
 }
 
 // ---------------------------------------------------------------------------
+// Star stencils from a register ring.  A star point reads its own columns in
+// rows o-R..o+R and its own row in columns -R..R.  The sweep for the larger
+// star radii keeps the lane's own columns of the last 2R+1 rows in registers,
+// so its LDS ring holds only the R+1 rows that will be, or now are, the
+// centre row: half the LDS per wave, so more waves fit on a CU (the 2R+1-row
+// ring held star R=5 and R=8 to 3 and 2 workgroups per CU).  The arithmetic
+// is that of compute_points, operand for operand.  Measured on MI355X at
+// n=16384 against the LDS ring: FP64 R=6/7/8 +2/+2/+4 %, FP32 R=5..8 +2..+6 %,
+// BF16 R=6/7/8 +13/+15/+17 %; below those radii it lost 1-4 % (the unrolled
+// ring costs more than the occupancy gains), so they keep the LDS ring.
+// ---------------------------------------------------------------------------
+template <int DT, int P, int R>
+constexpr bool star_reg_ring() { return P == STAR && R >= (DT == F32 ? 5 : 6); }
+
+// LDS ring rows per wave of the sweep kernel, and its static LDS per workgroup.
+template <int DT, int P, int R>
+constexpr int sweep_ring_rows() { return star_reg_ring<DT, P, R>() ? R + 1 : 2 * R + 1; }
+
+template <int DT, int P, int R>
+constexpr size_t sweep_lds_bytes()
+{
+    return (size_t)kWavesPerBlock * sweep_ring_rows<DT, P, R>() * SweepGeom<DT, R>::LR *
+           sizeof(typename DTypeTraits<DT>::T);
+}
+
+// FP64 tap K of both chains: horizontal taps read the centre-row window wc
+// (columns C0 .. C0+NWC-1 of the lane), vertical taps the register column.
+template <int R, int C0, int NWC, int K>
+__device__ inline void star_tap_f64(const double (&wc)[NWC], const double (&col)[2 * R + 1][2], double (&s)[2])
+{
+#pragma clang fp contract(off)
+    constexpr Tap t = Stencil<STAR, R>::taps.t[K];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        double x;
+        if constexpr (t.di == 0)
+            x = wc[v + t.dj - C0];
+        else
+            x = col[t.di + R][v];
+        if constexpr (K == 0)
+            s[v] = x * t.w;
+        else
+            s[v] = s[v] + x * t.w;
+    }
+}
+
+template <int R, int C0, int NWC, size_t... I>
+__device__ inline void star_taps_f64(const double (&wc)[NWC], const double (&col)[2 * R + 1][2], double (&s)[2],
+                                     std::index_sequence<I...>)
+{
+    (star_tap_f64<R, C0, NWC, (int)I>(wc, col, s), ...);
+}
+
+// Reduced-precision pair K (pair order of make_pairs): horizontal pairs from
+// the widened centre-row window, vertical pairs from the register columns.
+template <typename T, int R, int V, int HP, int K>
+__device__ inline void star_pair_step(const float (&wc)[V + 2 * HP], const T (&col)[2 * R + 1][V], float (&acc)[V])
+{
+#pragma clang fp contract(off)
+    constexpr Tap t = Stencil<STAR, R>::pairs.t[K];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        float xp, xm;
+        if constexpr (t.di == 0) {
+            xp = wc[HP + v + t.dj];
+            xm = wc[HP + v - t.dj];
+        } else {
+            xp = to_f32<T>(col[R + t.di][v]);
+            xm = to_f32<T>(col[R - t.di][v]);
+        }
+        if constexpr (K == 0)
+            acc[v] = (float)t.w * (xp - xm);
+        else
+            acc[v] = __builtin_fmaf((float)t.w, xp - xm, acc[v]);
+    }
+}
+
+template <typename T, int R, int V, int HP, size_t... I>
+__device__ inline void star_pairs(const float (&wc)[V + 2 * HP], const T (&col)[2 * R + 1][V], float (&acc)[V],
+                                  std::index_sequence<I...>)
+{
+    (star_pair_step<T, R, V, HP, (int)I>(wc, col, acc), ...);
+}
+
+// The V outputs of one lane for output row o: crow points at the LDS copy of
+// row o at the lane's first column, col[k] holds the lane's columns of row
+// o-R+k.
+template <int DT, int R, int V, int HP>
+__device__ inline void star_points(const typename DTypeTraits<DT>::T *crow,
+                                   const typename DTypeTraits<DT>::T (&col)[2 * R + 1][V],
+                                   const typename DTypeTraits<DT>::T *outv, typename DTypeTraits<DT>::T *res)
+{
+#pragma clang fp contract(off)
+    typedef typename DTypeTraits<DT>::T T;
+    if constexpr (DT == F64) {
+        static_assert(V == 2, "FP64 column pairs");
+        constexpr int C0 = (R & 1) ? -R - 1 : -R;          // aligned pairs cover columns -R .. R+1
+        constexpr int NWC = ((R & 1) ? R + 2 : R + 1) - C0 + 1;
+        double wc[NWC];
+#pragma unroll
+        for (int q = 0; q < NWC / 2; ++q) {
+            union { u32x4 v; double e[2]; } u;
+            u.v = *(const u32x4 *)(crow + C0 + 2 * q);
+            wc[2 * q] = u.e[0];
+            wc[2 * q + 1] = u.e[1];
+        }
+        double sum[2];
+        star_taps_f64<R, C0, NWC>(wc, col, sum, std::make_index_sequence<Stencil<STAR, R>::taps.count>{});
+#pragma unroll
+        for (int v = 0; v < V; ++v) res[v] = outv[v] + sum[v];
+    } else {
+        float wc[V + 2 * HP];
+        load_window<T, V, HP>(crow, wc);
+        float acc[V];
+        star_pairs<T, R, V, HP>(wc, col, acc, std::make_index_sequence<Stencil<STAR, R>::pairs.count>{});
+        float f[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) f[v] = to_f32<T>(outv[v]) + acc[v];
+        from_f32_vec<V>(f, res);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Raw buffer access.  Every HBM access of the sweep is a buffer instruction on
 // a per-row resource: an offset at or beyond the resource's byte range reads
 // zero and drops the write without touching memory.  Invalid rows, columns
@@ -580,6 +703,137 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
     }
 }
 
+// wave_tile for the star radii of star_reg_ring: the same loads, stores and
+// arithmetic, with the rows' main vectors held in a register ring qm of M
+// slots (M >= 2R+1+PF, a multiple of PF; row ri in slot (ri - lo) % M; the
+// row loop is unrolled by M so every slot index is a constant) and an LDS
+// ring of R+1 rows for the centre rows.
+template <int DT, int R, bool FUSE, int MODE, int PF, int NT>
+__device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                      typename DTypeTraits<DT>::T *ring_base, int lane)
+{
+    typedef SweepGeom<DT, R> G;
+    typedef typename G::T T;
+    constexpr int V = G::V, HP = G::HP, NH = G::NH, LR = G::LR, S = G::S;
+    constexpr int L = R + 1;                         // LDS ring rows
+    constexpr int M = (S + PF + PF - 1) / PF * PF;   // register ring slots, >= S + PF
+    constexpr int ES = (int)sizeof(T);
+    constexpr int AUX_STREAM = (NT & 1) ? 2 : 0;
+
+    const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
+    const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
+
+    const T *in = (const T *)in_;
+    T *in_next = (T *)in_next_;
+    T *out = (T *)out_;
+    const unsigned ld_bytes = (unsigned)(pitch * ES);
+    const unsigned st_bytes = pad_writable ? ld_bytes : (unsigned)(n * ES);
+
+    const int cm = c0 + lane * V;
+    const int ch = lane < NH ? c0 - HP + lane * V : c0 + G::SW + (lane - NH) * V;
+    const int lh = lane < NH ? lane * V : HP + G::SW + (lane - NH) * V;
+    const unsigned off_main = (unsigned)(cm * ES);
+    const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;
+    bool interior[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) interior[v] = (cm + v >= R) && (cm + v < n - R);
+
+    T(*ring)[LR] = (T(*)[LR])ring_base;
+
+    // Row ri: its main vector in qm[(ri - lo) % M], its halo vector and the
+    // out row ri - R in buf[(ri - lo) % PF] (M is a multiple of PF).
+    union Vec { u32x4 v; T e[V]; };
+    Vec qm[M];
+    struct Side { Vec halo, outv; } buf[PF];
+    auto issue = [&](Vec &m, Side &sd, int ri) {
+        const bool row_ok = ri < hi;
+        const int rr = row_ok ? ri : lo;
+        const __amdgpu_buffer_rsrc_t rin = row_rsrc(in + (long long)rr * pitch, ld_bytes);
+        const int o = ri - R;
+        const bool out_ok = o >= olo && o < ohi;
+        const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, ld_bytes);
+        if constexpr (MODE != MODE_ELEM) {
+            m.v = bload16<0>(rin, row_ok ? off_main : kOOB);
+            sd.halo.v = bload16<0>(rin, row_ok ? off_halo : kOOB);
+            sd.outv.v = bload16<AUX_STREAM>(rout, out_ok ? off_main : kOOB);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                m.e[v] = bload_elem<T, 0>(rin, row_ok ? off_main + v * ES : kOOB);
+                sd.halo.e[v] = bload_elem<T, 0>(rin, (row_ok && off_halo != kOOB) ? off_halo + v * ES : kOOB);
+                sd.outv.e[v] = bload_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB);
+            }
+        }
+    };
+
+#pragma unroll
+    for (int k = 0; k < PF; ++k) issue(qm[k], buf[k], lo + k);
+
+    int lslot = 0;  // LDS ring slot of row ri: (ri - lo) % L
+    for (int base = lo; base < hi; base += M) {
+#pragma clang loop unroll(full)
+        for (int k = 0; k < M; ++k) {
+            const int ri = base + k;
+            if (ri >= hi) return;  // uniform: the same row in every lane
+            __builtin_amdgcn_sched_barrier(0);
+            const Vec cur = qm[k];
+            const Side sd = buf[k % PF];
+            // qm slot (k+PF)%M held row ri+PF-M <= ri-2R-1, no longer needed.
+            issue(qm[(k + PF) % M], buf[k % PF], ri + PF);
+
+            *(u32x4 *)&ring[lslot][HP + lane * V] = cur.v;
+            if (lane < 2 * NH) *(u32x4 *)&ring[lslot][lh] = sd.halo.v;
+
+            if constexpr (FUSE) {
+                const bool nx_ok = ri >= ulo && ri < uhi;
+                const __amdgpu_buffer_rsrc_t rnx =
+                    row_rsrc(in_next + (long long)(nx_ok ? ri : r0) * pitch, st_bytes);
+                Vec nx;
+                plus_one<DT, V>(cur.e, nx.e);
+                if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rnx, nx_ok ? off_main : kOOB, nx.v);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        bstore_elem<T, AUX_STREAM>(rnx, nx_ok ? off_main + v * ES : kOOB, nx.e[v]);
+                }
+            }
+
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+            // Output row o = ri - R (rows o-R .. o+R are all >= lo here, see
+            // olo): its LDS slot is (lslot - R) mod (R+1) = lslot + 1 mod L,
+            // overwritten only by the next row; row o-R+j is in register
+            // slot (k - 2R + j) mod M.
+            const int o = ri - R;
+            if (o >= olo && o < ohi) {
+                const int cs = lslot + 1 == L ? 0 : lslot + 1;
+                T col[S][V];
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) col[j][v] = qm[(k + M - 2 * R + j) % M].e[v];
+                Vec res;
+                star_points<DT, R, V, HP>(&ring[cs][HP + lane * V], col, sd.outv.e, res.e);
+#pragma unroll
+                for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : sd.outv.e[v];
+                const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
+                if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_main + v * ES, res.e[v]);
+                }
+            }
+            lslot = lslot + 1 == L ? 0 : lslot + 1;
+        }
+    }
+}
+
 
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
 __global__ __launch_bounds__(kThreads)
@@ -590,7 +844,7 @@ void sweep_kernel(SweepArgs a)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
-    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][G::S][G::LR];
+    __shared__ __attribute__((aligned(16))) T lds[kWavesPerBlock][sweep_ring_rows<DT, P, R>()][G::LR];
 
     int group, rb;
     if (!tile_of(a, blockIdx.x, group, rb)) return;
@@ -600,8 +854,12 @@ void sweep_kernel(SweepArgs a)
     if (strip >= a.nstrips) return;
     const int r0 = rb * a.rows_per_wave;
     const int r1 = min(r0 + a.rows_per_wave, a.span.nrows);
-    wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW, r0,
-                                               r1, a.pad_writable, &lds[wave][0][0], lane);
+    if constexpr (star_reg_ring<DT, P, R>())
+        wave_tile_star<DT, R, FUSE, MODE, PF, NT>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW, r0,
+                                                  r1, a.pad_writable, &lds[wave][0][0], lane);
+    else
+        wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW,
+                                                   r0, r1, a.pad_writable, &lds[wave][0][0], lane);
 }
 
 }  // namespace prk

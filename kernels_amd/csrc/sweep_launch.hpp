@@ -40,7 +40,8 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
     }
     // Extra dynamic LDS caps the workgroups resident per CU (occupancy knob).
     const size_t lds_cap = p.wgs_per_cu > 0 ? (160u * 1024u) / (unsigned)p.wgs_per_cu : 0;
-    const size_t dyn = lds_cap > G::lds_bytes ? lds_cap - G::lds_bytes : 0;
+    constexpr size_t lds_bytes = sweep_lds_bytes<DT, P, R>();
+    const size_t dyn = lds_cap > lds_bytes ? lds_cap - lds_bytes : 0;
     hipLaunchKernelGGL((sweep_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
     return check_hip(hipGetLastError(), "sweep launch");
 }

@@ -251,3 +251,32 @@ def test_extended_radius_bitwise(gpu, oracle, pattern, radius, n):
         K.run(pattern, radius, dtype, iters, b0, b1, out)
         lp_out, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 5)
         assert np.array_equal(to_host(out).view(lp_out.dtype), lp_out), dtype
+
+
+# Star radii that sweep from a register ring (FP64 R >= 6, FP32 R >= 5,
+# BF16 R >= 6): every row-block height from one row up, each pipeline depth,
+# aligned and unaligned pitch, bitwise against the oracle.
+RING = [(K.F64, 6), (K.F64, 8), (F32, 5), (BF16, 7)]
+
+
+@pytest.mark.parametrize("dtype,radius", RING)
+def test_star_register_ring_tilings(gpu, oracle, dtype, radius):
+    n, iters, seed = 203, 3, 21
+    L = K.lib()
+    if dtype == K.F64:
+        ref, _ = oracle.run(0, radius, n, 32, iters, 1, seed)
+    else:
+        ref, _ = oracle.run_lp(dtype, 0, radius, n, iters, 1, seed)
+    try:
+        for v, rows, pitch in [(4, 1, None), (4, 2, None), (1, 5, n), (4, 13, None), (0, 40, None),
+                               (2, 203, n), (5, 7, None), (3, 3, n)]:
+            print(f"ring dtype={dtype} star{radius} variant={v} rows={rows} pitch={pitch}", flush=True)
+            K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
+            b0, b1, out = (K.alloc_grid(n, dtype, pitch=pitch) for _ in range(3))
+            K.init(K.INIT_SEEDED, seed, dtype, b0, out)
+            K.run(0, radius, dtype, iters, b0, b1, out)
+            got = to_host(out)
+            assert np.array_equal(bits(got), bits(ref)) if dtype == K.F64 else \
+                np.array_equal(got.view(ref.dtype), ref), (v, rows, pitch)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
