@@ -99,6 +99,8 @@ def uneven_bounds(n, parts, radius, seed):
 CASES = [
     (0, 1, F64), (0, 2, F64), (1, 2, F64), (0, 5, F64), (1, 3, F64), (0, 8, F64),
     (0, 2, F32), (1, 4, F32), (0, 2, BF16), (1, 1, BF16),
+    # star radii swept from a register ring
+    (0, 6, F64), (0, 5, F32), (0, 7, BF16),
 ]
 
 
@@ -110,7 +112,7 @@ def test_uneven_slabs_equal_whole_grid(gpu, pattern, radius, dtype):
     check_against_whole(pattern, radius, dtype, n, bounds, sweeps=4, init=1, seed=31)
 
 
-@pytest.mark.parametrize("radius", [1, 2, 4])
+@pytest.mark.parametrize("radius", [1, 2, 4, 6])
 def test_thin_slabs(gpu, radius):
     # slabs of exactly R rows (the thinnest the exchange allows), slabs that
     # lie inside the top and bottom frame, and one whole-grid slab
