@@ -280,3 +280,30 @@ def test_star_register_ring_tilings(gpu, oracle, dtype, radius):
                 np.array_equal(got.view(ref.dtype), ref), (v, rows, pitch)
     finally:
         K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
+
+
+UNFUSED = [(0, 2), (1, 3), (0, 5), (0, 7), (1, 6)]
+
+
+@pytest.mark.parametrize("dtype", [K.F64, F32, BF16])
+@pytest.mark.parametrize("pattern,radius", UNFUSED)
+def test_unfused_forms_equal_fused_sweep(gpu, dtype, pattern, radius):
+    # prk_stencil_ex (out += S(in)) then prk_add_ex (in += 1), in any dtype,
+    # give bit for bit what one fused sweep gives (the fused sweep is pinned
+    # to the oracle above); pitch n covers the element-access path.
+    L = K.lib()
+    for n, pitch in [(131, None), (131, 131)]:
+        print(f"unfused dtype={dtype} {NAME[pattern]}{radius} n={n} pitch={pitch}", flush=True)
+        b0, b1, out = (K.alloc_grid(n, dtype, pitch=pitch) for _ in range(3))
+        K.init(K.INIT_SEEDED, 33, dtype, b0, out)
+        xs = K.alloc_grid(n, dtype, pitch=pitch)
+        os_ = K.alloc_grid(n, dtype, pitch=pitch)
+        xs.copy_(b0)
+        os_.copy_(out)
+        K.sweep(pattern, radius, dtype, b0, b1, out)
+        stream = torch.cuda.current_stream()
+        K.stencil_ex(pattern, radius, dtype, xs, os_, stream)
+        K.check(L.prk_add_ex(dtype, n, xs.stride(0), xs.data_ptr(), stream.cuda_stream))
+        torch.cuda.synchronize()
+        assert np.array_equal(bits(to_host(os_)), bits(to_host(out)))
+        assert np.array_equal(bits(to_host(xs)), bits(to_host(b1)))
