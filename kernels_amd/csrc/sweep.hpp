@@ -12,9 +12,12 @@
 // is read from HBM once per strip with one 16-byte load per lane (plus a
 // handful of lanes loading the R-wide column halo), parked in a ring of 2R+1
 // rows in LDS, and every tap of the output row R rows above is then read
-// from LDS.  Rows are software-pipelined: the loads of the next rows are in
-// flight while the current row is computed.  A workgroup is four waves on
-// adjacent strips.  The workgroup -> tile order is a launch parameter
+// from LDS (star stencils at large R keep their vertical taps in a register
+// ring instead and park only R+1 rows in LDS: wave_tile_star).  FP64 lanes
+// read their LDS windows as aligned 16-byte column pairs, which keeps the
+// LDS banks free of conflicts at odd R.  Rows are software-pipelined: the
+// loads of the next rows are in flight while the current row is computed.  A
+// workgroup is four waves on adjacent strips.  The workgroup -> tile order is a launch parameter
 // (tile_of): row-major over the grid measured best on MI355X; the orders
 // that give each of the eight XCDs a contiguous run of tiles, so that a
 // tile's vertical neighbours (whose rows it re-reads as halo) share its L2,
