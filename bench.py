@@ -40,6 +40,8 @@ WORKLOADS = {
     # FP64 grid stencils bound by their tap chains (radius/precision sweep sizes)
     "grid3_f64_n16384": (1, 3, 0, 16384),
     "grid5_f64_n16384": (1, 5, 0, 16384),
+    # a star stencil swept from the register ring
+    "star8_f64_n16384": (0, 8, 0, 16384),
 }
 DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 BATCH_WORKLOADS = ("batch4096_f32",)
