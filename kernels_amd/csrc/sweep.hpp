@@ -127,8 +127,7 @@ struct SweepGeom {
     static constexpr int HP = ((R + V - 1) / V) * V;       // halo pad, whole vectors
     static constexpr int NH = HP / V;                      // halo vectors per side
     static constexpr int LR = SW + 2 * HP;                 // LDS row length (elements)
-    static constexpr int S = 2 * R + 1;                    // ring rows
-    static constexpr size_t lds_bytes = (size_t)kWavesPerBlock * S * LR * sizeof(T);
+    static constexpr int S = 2 * R + 1;                    // rows a point reads
 };
 
 template <typename T>
