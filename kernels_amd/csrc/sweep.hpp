@@ -779,7 +779,6 @@ __device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out
         for (int k = 0; k < M; ++k) {
             const int ri = base + k;
             if (ri >= hi) return;  // uniform: the same row in every lane
-            __builtin_amdgcn_sched_barrier(0);
             const Vec cur = qm[k];
             const Side sd = buf[k % PF];
             // qm slot (k+PF)%M held row ri+PF-M <= ri-2R-1, no longer needed.
