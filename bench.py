@@ -42,6 +42,8 @@ WORKLOADS = {
     "grid5_f64_n16384": (1, 5, 0, 16384),
     # a star stencil swept from the register ring
     "star8_f64_n16384": (0, 8, 0, 16384),
+    # the slowest stencil of the radius/precision sweep's table
+    "grid3_bf16_n16384": (1, 3, 2, 16384),
 }
 DT_NAME = {0: "f64", 1: "f32", 2: "bf16"}
 BATCH_WORKLOADS = ("batch4096_f32",)

@@ -282,20 +282,45 @@ __device__ inline void pair_sums_f64(const double *const (&rows)[2 * R + 1], dou
     if constexpr (C + 2 <= c_hi) pair_sums_f64<P, R, C + 2>(rows, s);
 }
 
-// Reduced-precision forms.  Each row a lane reads is fetched from LDS as whole
-// 16-byte vectors covering columns [-HP, V+HP) around its own V columns and
-// widened to float once; the paired sum then reads that register window.
-template <typename T, int V, int HP>
+// Reduced-precision forms.  The window of a row is the columns [-HP, V+HP)
+// around the lane's own V columns (window index i = column i - HP); a row
+// whose taps reach MAXD columns to either side needs the indices
+// [HP-MAXD, HP+V-1+MAXD].  Each 16-byte vector holding a needed column is
+// read from LDS and widened to float once; the paired sum then reads the
+// register window.  With WHOLE, an empty asm keeps every read whole: left to
+// itself the compiler narrows the partly used edge vectors to 2- and 4-byte
+// LDS reads at a 16-byte lane stride, which conflicted in the LDS banks for
+// 18.7 % of the BF16 grid R=3 sweep's cycles.  Measured on MI355X in
+// alternating runs, whole reads made BF16 grid R=5 16-18 % faster, but BF16
+// grid R=2, 3, 4 and 8 0-5 % slower (the asm also constrains the
+// scheduler), FP32 grid R=5 10 % slower and BF16 star R=5 no faster, so
+// only the BF16 grid R=5 sums use them.
+template <typename T, int V, int HP, int MAXD, bool WHOLE>
 __device__ inline void load_window(const T *row, float (&w)[V + 2 * HP])
 {
     constexpr int NV = (V + 2 * HP) / V;
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
+        if (q * V + V - 1 < HP - MAXD || q * V > HP + V - 1 + MAXD) continue;  // no needed column
         union { u32x4 v; T e[V]; } x;
         x.v = *(const u32x4 *)(row + (q * V - HP));
+        if constexpr (WHOLE) __asm__ volatile("" : "+v"(x.v));
 #pragma unroll
         for (int e = 0; e < V; ++e) w[q * V + e] = to_f32<T>(x.e[e]);
     }
+}
+
+// Widest column offset of the pairs on rows +-a.
+template <int P, int R>
+constexpr int pair_max_dj(int a)
+{
+    int m = 0;
+    for (int k = 0; k < Stencil<P, R>::pairs.count; ++k) {
+        const Tap t = Stencil<P, R>::pairs.t[k];
+        const int ad = t.di < 0 ? -t.di : t.di, aj = t.dj < 0 ? -t.dj : t.dj;
+        if (ad == a && aj > m) m = aj;
+    }
+    return m;
 }
 
 // acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs, in the pair order
@@ -341,13 +366,15 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
 {
     constexpr int b = pair_begin<P, R>(A), e = pair_begin<P, R>(A + 1);
     if constexpr (e > b) {
+        constexpr int md = pair_max_dj<P, R>(A);
+        constexpr bool whole = P == GRID && sizeof(T) == 2 && R == 5;
         float wp[V + 2 * HP];
-        load_window<T, V, HP>(rows[R + A], wp);
+        load_window<T, V, HP, md, whole>(rows[R + A], wp);
         if constexpr (A == 0) {
             pair_range<P, R, V, HP, b>(wp, wp, acc, std::make_index_sequence<e - b>{});
         } else {
             float wm[V + 2 * HP];
-            load_window<T, V, HP>(rows[R - A], wm);
+            load_window<T, V, HP, md, whole>(rows[R - A], wm);
             pair_range<P, R, V, HP, b>(wp, wm, acc, std::make_index_sequence<e - b>{});
         }
     }
@@ -499,7 +526,7 @@ __device__ inline void star_points(const typename DTypeTraits<DT>::T *crow,
         for (int v = 0; v < V; ++v) res[v] = outv[v] + sum[v];
     } else {
         float wc[V + 2 * HP];
-        load_window<T, V, HP>(crow, wc);
+        load_window<T, V, HP, R, false>(crow, wc);
         float acc[V];
         star_pairs<T, R, V, HP>(wc, col, acc, std::make_index_sequence<Stencil<STAR, R>::pairs.count>{});
         float f[V];
