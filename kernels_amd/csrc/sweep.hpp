@@ -291,10 +291,10 @@ __device__ inline void pair_sums_f64(const double *const (&rows)[2 * R + 1], dou
 // itself the compiler narrows the partly used edge vectors to 2- and 4-byte
 // LDS reads at a 16-byte lane stride, which conflicted in the LDS banks for
 // 18.7 % of the BF16 grid R=3 sweep's cycles.  Measured on MI355X in
-// alternating runs, whole reads made BF16 grid R=5 16-18 % faster, but BF16
-// grid R=2, 3, 4 and 8 0-5 % slower (the asm also constrains the
-// scheduler), FP32 grid R=5 10 % slower and BF16 star R=5 no faster, so
-// only the BF16 grid R=5 sums use them.
+// alternating runs, whole reads made BF16 grid R=5 16-18 % and R=7 8 %
+// faster, but BF16 grid R=2, 3, 4, 6 and 8 0-5 % slower (the asm also
+// constrains the scheduler), FP32 grid R=5 10 % slower and BF16 star R=5 no
+// faster, so only the BF16 grid R=5 and R=7 sums use them.
 template <typename T, int V, int HP, int MAXD, bool WHOLE>
 __device__ inline void load_window(const T *row, float (&w)[V + 2 * HP])
 {
@@ -367,7 +367,7 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
     constexpr int b = pair_begin<P, R>(A), e = pair_begin<P, R>(A + 1);
     if constexpr (e > b) {
         constexpr int md = pair_max_dj<P, R>(A);
-        constexpr bool whole = P == GRID && sizeof(T) == 2 && R == 5;
+        constexpr bool whole = P == GRID && sizeof(T) == 2 && (R == 5 || R == 7);
         float wp[V + 2 * HP];
         load_window<T, V, HP, md, whole>(rows[R + A], wp);
         if constexpr (A == 0) {
