@@ -535,10 +535,14 @@ int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override,
     void *mem = nullptr;
     double *d_norms = nullptr;
     if ((rc = check_hip(hipMalloc(&mem, 3 * need), "one-by-one buffers"))) return rc;
-    if ((rc = check_hip(hipMalloc(&d_norms, sizeof(double) * count), "one-by-one norms"))) {
+    // the norms, then this call's own norm partials (no workspace shared
+    // with other calls)
+    const size_t norms_bytes = align_up(sizeof(double) * count, 256);
+    if ((rc = check_hip(hipMalloc(&d_norms, norms_bytes + kNormPartialsBytes), "one-by-one norms"))) {
         (void)hipFree(mem);
         return rc;
     }
+    void *partials = (char *)d_norms + norms_bytes;
     hipStream_t st = (hipStream_t)stream;
     void *b0 = mem, *b1 = (char *)mem + need, *out = (char *)mem + 2 * need;
     for (int i = 0; i < count && rc == PRK_OK; ++i) {
@@ -550,7 +554,7 @@ int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override,
         void *bufs[2] = {b0, b1};
         for (int k = 0; k <= s.iterations && rc == PRK_OK; ++k)
             rc = launch_sweep(p, bufs[k & 1], bufs[(k + 1) & 1], out, st);
-        if (!rc) rc = l1norm_async(s.dtype, s.n, pitch, s.radius, out, d_norms + i, st);
+        if (!rc) rc = l1norm_async(s.dtype, s.n, pitch, s.radius, out, d_norms + i, partials, st);
     }
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "one-by-one sync");
     if (!rc && norms) rc = check_hip(hipMemcpy(norms, d_norms, sizeof(double) * count, hipMemcpyDeviceToHost), "norms");

@@ -199,8 +199,8 @@ __device__ inline DD dd_add(DD a, DD b)
     return DD{hi, lo - (hi - s)};
 }
 
-constexpr int kNormBlocks = 1024;
 constexpr int kNormThreads = 256;
+static_assert(sizeof(DD) * kNormBlocks == kNormPartialsBytes, "norm partials layout");
 
 template <int DT>
 __global__ __launch_bounds__(kNormThreads) void norm_partial_kernel(const void *out, int n,
@@ -346,15 +346,13 @@ __global__ __launch_bounds__(256) void copy_kernel(long long nvec, u32x4 *dst, c
     if (k < nvec) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[k]), &dst[k]);
 }
 
-int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, hipStream_t s)
+int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, void *partials,
+                 hipStream_t s)
 {
     if (!valid_dtype(dtype)) return set_error(PRK_EINVAL, "unknown dtype %d", dtype);
-    if (n < 1 || pitch < n || !out || !d_result || radius < 1 || 2 * radius + 1 > n)
+    if (n < 1 || pitch < n || !out || !d_result || !partials || radius < 1 || 2 * radius + 1 > n)
         return set_error(PRK_EINVAL, "bad norm arguments (n=%d radius=%d)", n, radius);
-    NormWorkspace *ws = nullptr;
-    int rc = norm_workspace(&ws);
-    if (rc) return rc;
-    return enqueue_l1norm(dtype, n, pitch, radius, out, d_result, ws->partials, s);
+    return enqueue_l1norm(dtype, n, pitch, radius, out, d_result, (DD *)partials, s);
 }
 
 }  // namespace prk

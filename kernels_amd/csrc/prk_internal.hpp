@@ -31,9 +31,15 @@ struct SweepPlan {
 // Number of fused-sweep variants (pipeline depth 1..3 x temporal/non-temporal).
 constexpr int kSweepVariants = 6;
 
-// L1 norm into a device double, enqueued on s (shares one per-device
-// workspace: norms on different streams must not overlap).
-int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, hipStream_t s);
+// The L1 norm's first pass leaves one double-double partial per block, at
+// most kNormBlocks of them, in a device buffer of kNormPartialsBytes.
+constexpr int kNormBlocks = 1024;
+constexpr size_t kNormPartialsBytes = 2 * sizeof(double) * kNormBlocks;
+
+// L1 norm into a device double, enqueued on s, with the caller's partials
+// buffer (kNormPartialsBytes of device memory, in use until the norm is done).
+int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, void *partials,
+                 hipStream_t s);
 
 // Tiling a sweep of this shape uses: tuned entry, nearest tuned size, or the
 // built-in default (autotune.hip).

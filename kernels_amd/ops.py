@@ -34,8 +34,63 @@ def _stream(stream) -> int | None:
     return getattr(stream, "cuda_stream", stream)
 
 
-def _pitch(t: torch.Tensor) -> int:
-    return t.stride(0) if t.dim() == 2 else t.shape[-1]
+def _check_grids(dtype: int, *grids, square: bool = True, rows: int | None = None) -> tuple[int, int]:
+    """Checks the buffers of one call before the kernels see them; returns
+    (rows, pitch).  Every grid must be a 2-D tensor of the storage type's
+    torch dtype with unit column stride; all share shape and row stride
+    (pitch >= columns); each one's storage must hold rows * pitch elements
+    from its first element (the kernels may read a whole padded row); and
+    they must live on a HIP device.  None entries (an absent out) are
+    skipped.  Raises ValueError (TypeError for a non-tensor)."""
+    if dtype not in _TORCH:
+        raise ValueError(f"unknown dtype {dtype}")
+    want, es = _TORCH[dtype], _ESIZE[dtype]
+    ts = [t for t in grids if t is not None]
+    if not ts:
+        raise ValueError("no grid given")
+    for t in ts:
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"grid must be a torch tensor, got {type(t).__name__}")
+        if t.dtype != want:
+            raise ValueError(f"grid of dtype {t.dtype} passed for storage type {dtype} ({want})")
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise ValueError(f"grid must be 2-D with unit column stride, got shape {tuple(t.shape)} "
+                             f"strides {t.stride()}")
+    ref = ts[0]
+    for t in ts[1:]:
+        if t.shape != ref.shape or t.stride(0) != ref.stride(0):
+            raise ValueError(f"grids differ in shape or row stride: {tuple(ref.shape)}/{ref.stride(0)} vs "
+                             f"{tuple(t.shape)}/{t.stride(0)}")
+    nrows, ncols = ref.shape
+    if square and nrows != ncols:
+        raise ValueError(f"grid must be square, got {tuple(ref.shape)}")
+    if rows is not None and nrows != rows:
+        raise ValueError(f"grid has {nrows} rows, expected {rows}")
+    pitch = ref.stride(0) if nrows > 1 else max(ref.stride(0), ncols)
+    if pitch < ncols:
+        raise ValueError(f"row stride {pitch} < {ncols} columns")
+    need = nrows * pitch * es
+    for t in ts:
+        avail = t.untyped_storage().nbytes() - t.storage_offset() * es
+        if avail < need:
+            raise ValueError(f"grid storage holds {avail} bytes from its first element; the kernels "
+                             f"address {need} ({nrows} rows of pitch {pitch})")
+    for t in ts:
+        if not t.is_cuda:
+            raise ValueError("grid must be a HIP device tensor (there is no CPU path)")
+        if t.device != ref.device:
+            raise ValueError("grids are on different devices")
+    return nrows, pitch
+
+
+def _check_dense(n: int, *grids) -> None:
+    """The drop-in operators' buffers: n x n FP64 with pitch n (the
+    reference's prk::vector layout)."""
+    for t in grids:
+        if isinstance(t, torch.Tensor) and (tuple(t.shape) != (n, n) or not t.is_contiguous()):
+            raise ValueError(f"drop-in operators take a contiguous {n} x {n} FP64 grid, got shape "
+                             f"{tuple(t.shape)} strides {t.stride()}")
+    _check_grids(F64, *grids)
 
 
 def alloc_grid(n: int, dtype: int = F64, device: str = "cuda", pitch: int | None = None,
@@ -54,6 +109,7 @@ def alloc_grid(n: int, dtype: int = F64, device: str = "cuda", pitch: int | None
 
 def stencil(name_or_pattern, n: int, t: int, inp: torch.Tensor, out: torch.Tensor, radius: int | None = None):
     """Drop-in operator: out += S(in), FP64, pitch n, null stream."""
+    _check_dense(n, inp, out)
     if isinstance(name_or_pattern, str):
         pattern, radius = OPERATORS[name_or_pattern]
         check(getattr(lib(), f"prk_{name_or_pattern}")(n, t, inp.data_ptr(), out.data_ptr()))
@@ -62,13 +118,14 @@ def stencil(name_or_pattern, n: int, t: int, inp: torch.Tensor, out: torch.Tenso
 
 
 def add(n: int, t: int, inp: torch.Tensor) -> None:
+    _check_dense(n, inp)
     check(lib().prk_add(n, t, inp.data_ptr()))
 
 
 def sweep(pattern: int, radius: int, dtype: int, inp: torch.Tensor, in_next: torch.Tensor,
           out: torch.Tensor, stream=None) -> None:
-    n = inp.shape[0]
-    check(lib().prk_sweep(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(in_next), _ptr(out),
+    n, pitch = _check_grids(dtype, inp, in_next, out)
+    check(lib().prk_sweep(pattern, radius, dtype, n, pitch, _ptr(inp), _ptr(in_next), _ptr(out),
                           _stream(stream)))
 
 
@@ -94,9 +151,10 @@ def sweep_slab(pattern: int, radius: int, dtype: int, n: int, row0: int, rows: i
     n x n grid (prk_sweep_slab): buffers hold those rows and their halo rows."""
     top, bot = slab_halos(n, radius, row0, rows)
     for t in (inp, in_next, out):
-        if t.shape[0] != top + rows + bot or t.shape[1] != n:
+        if t.dim() != 2 or t.shape[0] != top + rows + bot or t.shape[1] != n:
             raise ValueError(f"slab buffer of shape {tuple(t.shape)}, expected ({top + rows + bot}, {n})")
-    check(lib().prk_sweep_slab(pattern, radius, dtype, n, row0, rows, _pitch(inp), _ptr(inp), _ptr(in_next),
+    _, pitch = _check_grids(dtype, inp, in_next, out, square=False)
+    check(lib().prk_sweep_slab(pattern, radius, dtype, n, row0, rows, pitch, _ptr(inp), _ptr(in_next),
                                _ptr(out), _stream(stream)))
 
 
@@ -107,27 +165,27 @@ def autotune(pattern: int, radius: int, dtype: int, buf0: torch.Tensor, buf1: to
     sweeps.  Returns the chosen Tuning."""
     from ._lib import Tuning
     t = Tuning()
-    n = buf0.shape[0]
-    check(lib().prk_autotune_buffers(pattern, radius, dtype, n, _pitch(buf0), _ptr(buf0), _ptr(buf1), _ptr(out),
+    n, pitch = _check_grids(dtype, buf0, buf1, out)
+    check(lib().prk_autotune_buffers(pattern, radius, dtype, n, pitch, _ptr(buf0), _ptr(buf1), _ptr(out),
                                      reps, _stream(stream), ctypes.byref(t)))
     return t
 
 
 def stencil_ex(pattern: int, radius: int, dtype: int, inp: torch.Tensor, out: torch.Tensor, stream=None) -> None:
-    n = inp.shape[0]
-    check(lib().prk_stencil_ex(pattern, radius, dtype, n, _pitch(inp), _ptr(inp), _ptr(out), _stream(stream)))
+    n, pitch = _check_grids(dtype, inp, out)
+    check(lib().prk_stencil_ex(pattern, radius, dtype, n, pitch, _ptr(inp), _ptr(out), _stream(stream)))
 
 
 def init(kind: int, seed: int, dtype: int, inp: torch.Tensor, out: torch.Tensor | None, stream=None) -> None:
-    n = inp.shape[0]
-    check(lib().prk_init(kind, ctypes.c_uint64(seed), dtype, n, _pitch(inp), _ptr(inp), _ptr(out),
+    n, pitch = _check_grids(dtype, inp, out)
+    check(lib().prk_init(kind, ctypes.c_uint64(seed), dtype, n, pitch, _ptr(inp), _ptr(out),
                          _stream(stream)))
 
 
 def l1norm(dtype: int, radius: int, out: torch.Tensor, stream=None) -> float:
-    n = out.shape[0]
+    n, pitch = _check_grids(dtype, out)
     v = ctypes.c_double(0.0)
-    check(lib().prk_l1norm(dtype, n, _pitch(out), radius, _ptr(out), ctypes.byref(v), _stream(stream)))
+    check(lib().prk_l1norm(dtype, n, pitch, radius, _ptr(out), ctypes.byref(v), _stream(stream)))
     return v.value
 
 
@@ -135,10 +193,10 @@ def run(pattern: int, radius: int, dtype: int, iterations: int, buf0: torch.Tens
         out: torch.Tensor, stream=None) -> tuple[int, float]:
     """iterations+1 fused sweeps; returns (index of the buffer holding the
     final in, average seconds per timed sweep)."""
-    n = buf0.shape[0]
+    n, pitch = _check_grids(dtype, buf0, buf1, out)
     fb = ctypes.c_int(0)
     avg = ctypes.c_double(0.0)
-    check(lib().prk_run(pattern, radius, dtype, n, _pitch(buf0), iterations, _ptr(buf0), _ptr(buf1),
+    check(lib().prk_run(pattern, radius, dtype, n, pitch, iterations, _ptr(buf0), _ptr(buf1),
                         _ptr(out), _stream(stream), ctypes.byref(fb), ctypes.byref(avg)))
     return fb.value, avg.value
 
@@ -170,27 +228,37 @@ def nominal_flops(pattern: int, radius: int, n: int) -> int:
 # ---------------------------------------------------------------------------
 # Matrix transpose (B += A^T, A += 1), order x order FP64 row-major.
 # ---------------------------------------------------------------------------
+def _check_matrices(*mats) -> int:
+    """Transpose operands: contiguous order x order FP64 device tensors."""
+    order, _ = _check_grids(F64, *mats)
+    for m in mats:
+        if not m.is_contiguous():
+            raise ValueError(f"transpose operands must be contiguous, got strides {m.stride()}")
+    return order
+
+
 def transpose(A: torch.Tensor, B: torch.Tensor, stream=None) -> None:
-    order = A.shape[0]
+    order = _check_matrices(A, B)
     check(lib().prk_transpose_ex(order, _ptr(A), _ptr(B), _stream(stream)))
 
 
 def transpose_init(A: torch.Tensor, B: torch.Tensor, stream=None) -> None:
-    check(lib().prk_transpose_init(A.shape[0], _ptr(A), _ptr(B), _stream(stream)))
+    check(lib().prk_transpose_init(_check_matrices(A, B), _ptr(A), _ptr(B), _stream(stream)))
 
 
 def transpose_run(iterations: int, A: torch.Tensor, B: torch.Tensor, stream=None) -> float:
     """iterations + 1 transposes; returns the average seconds of the last
     `iterations`."""
     avg = ctypes.c_double(0.0)
-    check(lib().prk_transpose_run(A.shape[0], iterations, _ptr(A), _ptr(B), _stream(stream), ctypes.byref(avg)))
+    check(lib().prk_transpose_run(_check_matrices(A, B), iterations, _ptr(A), _ptr(B), _stream(stream),
+                                  ctypes.byref(avg)))
     return avg.value
 
 
 def transpose_check(iterations: int, B: torch.Tensor, stream=None) -> float:
     """The reference's validation sum of absolute differences (0 = exact)."""
     v = ctypes.c_double(0.0)
-    check(lib().prk_transpose_check(B.shape[0], iterations, _ptr(B), ctypes.byref(v), _stream(stream)))
+    check(lib().prk_transpose_check(_check_matrices(B), iterations, _ptr(B), ctypes.byref(v), _stream(stream)))
     return v.value
 
 

@@ -68,12 +68,20 @@ def test_serving_client_loop(tmp_path):
     assert "ERROR: AddressSanitizer" not in r.stderr
 
 
-@pytest.mark.parametrize("sanitizer,args", [("address", ["500", "2", "200", "0"]),
-                                            ("address", ["100", "1", "200", "1"]),
-                                            ("thread", ["200", "1", "200", "0"])])
-def test_server_host_logic(tmp_path, sanitizer, args):
+# gbs > 0: the stand-in's device takes time (bytes / gbs) and aborts when the
+# host frees or reads memory that work still in flight uses; at these rates
+# the served request stream saturates it, so the queue fills, requests wait
+# for pool memory and the window tables grow while windows run.
+@pytest.mark.parametrize("sanitizer,args,gbs", [("address", ["500", "2", "200", "0"], 0),
+                                                ("address", ["100", "1", "200", "1"], 0),
+                                                ("thread", ["200", "1", "200", "0"], 0),
+                                                ("address", ["2000", "3", "200", "0"], 6000),
+                                                ("thread", ["500", "2", "200", "0"], 6000)])
+def test_server_host_logic(tmp_path, sanitizer, args, gbs):
     exe = build(tmp_path, sanitizer)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", TSAN_OPTIONS="halt_on_error=1")
+    if gbs:
+        env["PRK_MOCK_GBS"] = str(gbs)
     r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=300, env=env)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr[-4000:]
