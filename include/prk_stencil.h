@@ -141,6 +141,20 @@ int prk_init(int init, uint64_t seed, int dtype, int n, long long pitch, void *i
 int prk_l1norm(int dtype, int n, long long pitch, int radius, const void *out, double *norm,
                void *stream);
 
+/* The norm of prk_l1norm, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels, with their 1024 double-double block partials in a static
+ * device array of 16 KiB per GPU instead of the library's workspace; the
+ * result (bitwise prk_l1norm's) goes to *d_norm, device memory the caller
+ * owns.  Enqueued on `stream` and not waited for: no allocation, no copy, no
+ * synchronization.  Every n fits (the norm uses at most 1024 partials; a
+ * request beyond the array would return PRK_EINVAL).  Calls on one GPU share
+ * its array: calls that overlap on one GPU (on different streams, or from
+ * different threads without ordering) race and give wrong norms. */
+int prk_l1norm_noalloc(int dtype, int n, long long pitch, int radius, const void *out, double *d_norm,
+                       void *stream);
+
 /* The driver loop of Cxx11/stencil.cc:183-191 on the device: iterations+1
  * fused sweeps over the ping-pong pair (buf0 holds the initial in; buf1 is
  * scratch), sweep 0 untimed, sweeps 1..iterations timed with HIP events on

@@ -31,6 +31,7 @@ SIGNATURES = {
     "prk_add_ex": (c_int, [c_int, c_int, c_ll, c_void_p, c_void_p]),
     "prk_init": (c_int, [c_int, c_u64, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p]),
     "prk_l1norm": (c_int, [c_int, c_int, c_ll, c_int, c_void_p, c_double_p, c_void_p]),
+    "prk_l1norm_noalloc": (c_int, [c_int, c_int, c_ll, c_int, c_void_p, c_void_p, c_void_p]),
     "prk_run": (c_int, [c_int, c_int, c_int, c_int, c_ll, c_int, c_void_p, c_void_p, c_void_p,
                         c_void_p, c_int_p, c_double_p]),
     "prk_set_rows_per_wave": (c_int, [c_int]),

@@ -277,11 +277,18 @@ static int norm_workspace(NormWorkspace **ws)
     return PRK_OK;
 }
 
+// Blocks of the norm's first pass (one partial each): one per interior row,
+// at most kNormBlocks.
+static int norm_blocks(int n, int radius)
+{
+    return (int)std::max<long long>(1, std::min<long long>(kNormBlocks, (long long)n - 2 * radius));
+}
+
 int enqueue_l1norm(int dtype, int n, long long pitch, int radius, const void *out, double *d_result,
                    DD *partials, hipStream_t s)
 {
     const long long w = n - 2 * radius;
-    const int blocks = (int)std::max<long long>(1, std::min<long long>(kNormBlocks, w));
+    const int blocks = norm_blocks(n, radius);
     switch (dtype) {
     case F64:
         hipLaunchKernelGGL(norm_partial_kernel<F64>, dim3(blocks), dim3(kNormThreads), 0, s, out, n, pitch, radius, partials);
@@ -344,6 +351,36 @@ __global__ __launch_bounds__(256) void copy_kernel(long long nvec, u32x4 *dst, c
 {
     const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
     if (k < nvec) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[k]), &dst[k]);
+}
+
+// ---------------------------------------------------------------------------
+// The norm's partials in a static device array, for the allocation-free
+// entry point.  In host code `scratch` names a host-side stand-in, not device
+// memory: kernels get the device address looked up for the current device
+// (each device holds its own copy of the array).
+// ---------------------------------------------------------------------------
+constexpr int kScratchPartials = kNormBlocks;  // 16 KiB per device
+static __device__ __align__(256) DD scratch[kScratchPartials];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+static int scratch_address(void **ptr)
+{
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return set_error(PRK_EINVAL, "device %d beyond the %d supported", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "norm scratch address");
+        if (rc) return rc;
+        if (!p) return set_error(PRK_EHIP, "norm scratch address is null");
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return PRK_OK;
 }
 
 int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out, double *d_result, void *partials,
@@ -501,6 +538,21 @@ int prk_l1norm(int dtype, int n, long long pitch, int radius, const void *out, d
     rc = check_hip(hipMemcpyAsync(norm, ws->result, sizeof(double), hipMemcpyDeviceToHost, s), "norm copy");
     if (rc) return rc;
     return check_hip(hipStreamSynchronize(s), "norm sync");
+}
+
+int prk_l1norm_noalloc(int dtype, int n, long long pitch, int radius, const void *out, double *d_norm, void *stream)
+{
+    if (!valid_dtype(dtype)) return set_error(PRK_EINVAL, "unknown dtype %d", dtype);
+    if (n < 1 || pitch < n || !out || !d_norm || radius < 1 || 2 * radius + 1 > n)
+        return set_error(PRK_EINVAL, "bad norm arguments (n=%d radius=%d)", n, radius);
+    const int blocks = norm_blocks(n, radius);
+    if (blocks > kScratchPartials)
+        return set_error(PRK_EINVAL, "norm of n=%d needs %d partials; the static scratch holds %d", n, blocks,
+                         kScratchPartials);
+    void *ptr = nullptr;
+    int rc = scratch_address(&ptr);
+    if (rc) return rc;
+    return enqueue_l1norm(dtype, n, pitch, radius, out, d_norm, (DD *)ptr, (hipStream_t)stream);
 }
 
 int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iterations, void *buf0, void *buf1,
