@@ -154,3 +154,77 @@ def test_batch_autotune_keeps_results(gpu):
         tuned.close()
     finally:
         tuning_clear()
+
+
+def fp32_normwise_bound(iters: int) -> float:
+    """Normwise error of an FP32-storage solve against FP64 after iters+1
+    sweeps grows with the sweeps (the FP32 rounding of in_0 + k enters each
+    one): over the config-4 sample it measured 1.6e-7 at K=4, 6.4e-7 to
+    7.1e-7 at K=28..33 and 1.12e-6 at K=50 (oracle arithmetic, which the
+    kernels match bit for bit).  The bound is 1.3x to 1.9x those."""
+    return 3.5e-8 * (iters + 2)
+
+
+def test_config4_mix(gpu, oracle):
+    # The 4096-grid mix of the batched benchmark (seed 7: n log-uniform in
+    # [64, 2048], star/grid, R 1..4, K 1..50, FP32 storage, seeded U(-1,1)):
+    # every batched norm equals the one-grid-per-launch FP32 norm bit for
+    # bit; a spread sample (every 400th solve, the largest grid and the
+    # largest grid R=4 at K=50) equals the oracle bit for bit on three row
+    # bands (the whole grid up to n=600), in FP32 and in FP64; and each
+    # sampled FP32 result is within the stated bound of its FP64 result.
+    import bench
+    from concurrent.futures import ThreadPoolExecutor
+
+    specs = bench.config4_solves()
+    big = max(range(len(specs)), key=lambda i: specs[i][3])
+    big_g4 = max((i for i, s in enumerate(specs) if s[0] == 1 and s[1] == 4), key=lambda i: specs[i][3] * specs[i][4])
+    sample = list(range(0, len(specs), 400)) + [big, big_g4]
+    assert specs[big][3] >= 2000 and specs[big_g4][1] == 4 and specs[big_g4][4] == 50
+    print("config 4: batch of 4096", flush=True)
+    b = Batch(specs)
+    b.run()
+    norms = b.wait()
+    outs = {i: (b.output(i, 0), b.output(i, 1)) for i in sample}
+    b.close()
+    print("config 4: one grid per launch", flush=True)
+    n32 = one_by_one(specs)
+    assert np.array_equal(norms, n32), int(np.sum(norms != n32))
+    print("config 4: FP64 sample", flush=True)
+    s64 = [(specs[i][0], specs[i][1], F64) + tuple(specs[i][3:]) for i in sample]
+    b64 = Batch(s64)
+    b64.run()
+    b64.wait()
+    outs64 = [(b64.output(j, 0), b64.output(j, 1)) for j in range(len(sample))]
+    b64.close()
+
+    def spans(s):
+        n, r = s[3], s[1]
+        if n <= 600:
+            return [(0, n)]
+        return [(0, r + 6), (n // 2 - 4, n // 2 + 4), (n - r - 6, n)]
+
+    def check(j):
+        i = sample[j]
+        p, r, d, n, k, init, seed = specs[i]
+        msgs = []
+        for a, bb in spans(specs[i]):
+            lp = oracle.band_lp(F32, p, r, n, k, init, seed, a, bb)
+            if not np.array_equal(outs[i][0][a:bb].view(np.uint32), lp.view(np.uint32)):
+                msgs.append(f"FP32 band {a}:{bb} of {specs[i]}")
+            f64 = oracle.band(p, r, n, k, init, seed, a, bb)
+            if not np.array_equal(outs64[j][0][a:bb].view(np.uint64), f64.view(np.uint64)):
+                msgs.append(f"FP64 band {a}:{bb} of {specs[i]}")
+        return msgs
+
+    with ThreadPoolExecutor(8) as ex:
+        bad = [m for ms in ex.map(check, range(len(sample))) for m in ms]
+    assert not bad, bad
+    for j, i in enumerate(sample):
+        r, n, k = specs[i][1], specs[i][3], specs[i][4]
+        inner = (slice(r, n - r),) * 2
+        y = outs64[j][0][inner]
+        dd = np.abs(outs[i][0].astype(np.float64)[inner] - y)
+        normwise = float(np.linalg.norm(dd) / np.linalg.norm(y))
+        print(f"config 4 solve {i} {specs[i]}: FP32 normwise {normwise:.3e} max abs {dd.max():.3e}", flush=True)
+        assert normwise <= fp32_normwise_bound(k), (specs[i], normwise)

@@ -1,12 +1,15 @@
 """The full-size workloads, checked where the oracle can reach them.
 
 * n=30000, 100 iterations, FP64 star R=2 and grid R=2: the analytic run
-  validates (|norm - 2(K+1)| <= 1e-8, and for star R=2, whose weights are
-  dyadic, every interior point equals 2(K+1) exactly); the seeded U(-1,1)
-  run (seed 42) matches the oracle bitwise on row bands at the top, middle
-  and bottom of the grid.  A band of output rows [a,b) after K sweeps needs
-  only input rows [a-R, b+R): `in` after k sweeps is in_0 + k whatever `out`
-  holds, so the oracle computes bands of a grid it could not hold whole.
+  validates (|norm - 2(K+1)| <= 1e-8; for star R=2, whose weights are
+  dyadic, every interior point equals 2(K+1) exactly, and for grid R=2 every
+  interior point is within 1.5e-10 of it); the seeded U(-1,1) run (seed 42)
+  matches the oracle bitwise on 16 row bands at seeded random places, on
+  bands across the row-block boundaries of 6- and 8-row tiles (the first, a
+  middle and the last), and on the top and bottom frame rows.  A band of
+  output rows [a,b) after K sweeps needs only input rows [a-R, b+R): `in`
+  after k sweeps is in_0 + k whatever `out` holds, so the oracle computes
+  bands of a grid it could not hold whole.
 * n=16384, 50 iterations, star R=1..5 and grid R=1..3, analytic and seeded
   init, FP64 vs FP32 and BF16 storage (FP32 accumulation): every form
   matches its oracle bitwise on row bands, and the reduced-precision outputs
@@ -17,6 +20,8 @@
   radius, pattern and storage type.
 """
 import json
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -42,6 +47,28 @@ def bands(n: int, radius: int, h: int = 4):
     return [(0, radius + h), (mid - h // 2, mid + h // 2), (n - radius - h, n)]
 
 
+def wide_bands(n: int, radius: int, seed: int = 2026, count: int = 16, h: int = 8):
+    """Row bands for a point-wise check of a grid too large for the oracle:
+    `count` bands of h rows at seeded random places, bands of 6 rows across
+    the boundaries of 6- and 8-row tiles (the first, one in the middle, the
+    last inside the interior) and the frame rows at the top and bottom."""
+    rng = np.random.default_rng(seed)
+    out = [(int(a), int(a) + h) for a in sorted(rng.integers(0, n - h + 1, count))]
+    for rb in (6, 8):
+        for c in (rb, (n // 2) // rb * rb, (n - radius) // rb * rb):
+            out.append((max(0, c - 3), min(n, c + 3)))
+    out += [(0, radius + 4), (n - radius - 4, n)]
+    return out
+
+
+def oracle_bands(oracle, pattern, radius, n, iters, init, seed, spans):
+    """The oracle's bands, computed on the host's cores in parallel (the
+    oracle call releases the interpreter lock)."""
+    workers = max(1, min(16, os.cpu_count() or 1, len(spans)))
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(lambda ab: oracle.band(pattern, radius, n, iters, init, seed, *ab), spans))
+
+
 def solve(pattern, radius, dtype, n, iters, init, seed):
     b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
     K.init(init, seed, dtype, b0, out)
@@ -58,8 +85,14 @@ def test_n30000_fp64(gpu, oracle, pattern):
     out, fin = solve(pattern, radius, F64, n, iters, 0, 0)
     norm = K.l1norm(F64, radius, out)
     assert abs(norm - 2.0 * (iters + 1)) <= 1e-8
+    inner = out[radius:n - radius, radius:n - radius]
     if pattern == 0:
-        assert bool(torch.all(out[radius:n - radius, radius:n - radius] == 2.0 * (iters + 1)))
+        assert bool(torch.all(inner == 2.0 * (iters + 1)))
+    else:
+        # the weights of grid R=2 are not dyadic: each sweep rounds, by at
+        # most 9.3e-11 over the whole grid after 101 sweeps (oracle bands)
+        dev = float((inner - 2.0 * (iters + 1)).abs().max())
+        assert dev <= 1.5e-10, dev
     # in = i + j + (K+1) exactly
     i = torch.arange(n, device=gpu, dtype=torch.float64)
     assert bool(torch.all(fin[: 64] == (i[:64, None] + i[None, :] + iters + 1)))
@@ -67,8 +100,9 @@ def test_n30000_fp64(gpu, oracle, pattern):
     torch.cuda.empty_cache()
     print(f"n=30000 {NAME[pattern]}2 seeded 42", flush=True)
     out, fin = solve(pattern, radius, F64, n, iters, 1, 42)
-    for a, b in bands(n, radius):
-        ref = oracle.band(pattern, radius, n, iters, 1, 42, a, b)
+    spans = wide_bands(n, radius)
+    assert len(spans) >= 16 + 6 + 2
+    for (a, b), ref in zip(spans, oracle_bands(oracle, pattern, radius, n, iters, 1, 42, spans)):
         assert np.array_equal(host_rows(out, a, b).view(np.uint64), ref.view(np.uint64)), (a, b)
     del out, fin
     torch.cuda.empty_cache()
@@ -94,15 +128,19 @@ def errors(x: torch.Tensor, y: torch.Tensor, radius: int) -> dict:
                                                               torch.linalg.vector_norm(yd))}
 
 
-# Bounds on the normwise relative error against FP64 after 50 iterations.
-# FP32: analytic init keeps every value an integer below 2^24 or a sum of
-# rounded weights, so the error is a few FP32 rounding steps; seeded init
-# stores in_0 + k in FP32, whose rounding (ulp 2^-18 near 51) enters every
-# sweep's stencil.  BF16 (8-bit significand) stores in_0 + k to within
-# 2^-3 near 51 and i + j + k to within 64 near 16384: its error against FP64
-# is of the order of the result itself for both inits, so only finiteness
-# is required of it; its parity with the oracle is bitwise all the same.
-FP32_BOUND = {0: 1e-6, 1: 2e-5}
+# Bounds on the normwise relative error against FP64 after 50 iterations,
+# about 1.5x the largest value measured over the eight stencils.  FP32:
+# analytic init keeps every value an integer below 2^24 or a sum of rounded
+# weights, so the error is a few FP32 rounding steps (at most 7.5e-8,
+# star R=5 and grid R=3); seeded init stores in_0 + k in FP32, whose
+# rounding (ulp 2^-18 near 51) enters every sweep's stencil: 1.12e-6 to
+# 1.13e-6 in every case, above the 1e-6 the FP32 form aims for.  BF16
+# (8-bit significand) stores in_0 + k to within 2^-3 near 51 and i + j + k
+# to within 64 near 16384: its error against FP64 is of the order of the
+# result itself with analytic init (normwise 3.6 to 6.7) and 0.073 to 0.075
+# with seeded init; its parity with the oracle is bitwise all the same.
+FP32_BOUND = {0: 1.2e-7, 1: 1.7e-6}
+BF16_BOUND = {0: 10.0, 1: 0.11}
 
 
 @pytest.mark.parametrize("init,seed", [(0, 0), (1, 1234)])
@@ -130,10 +168,8 @@ def test_n16384_precision(gpu, oracle, pattern, radius, init, seed):
         print(json.dumps({"case": f"{NAME[pattern]}{radius}", "init": "analytic" if init == 0 else "seeded",
                           "dtype": {F32: "f32", BF16: "bf16"}[dtype], "n": n, "iterations": iters,
                           **{k: float(f"{v:.3e}") for k, v in e.items()}}), flush=True)
-        if dtype == F32:
-            assert e["normwise_l2"] <= FP32_BOUND[init], e
-        else:
-            assert all(np.isfinite(v) for v in e.values()), e
+        assert all(np.isfinite(v) for v in e.values()), e
+        assert e["normwise_l2"] <= (FP32_BOUND if dtype == F32 else BF16_BOUND)[init], e
         del out, fin
     del ref_out, ref_fin
     torch.cuda.empty_cache()

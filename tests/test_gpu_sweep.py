@@ -142,7 +142,10 @@ def test_storage_forms_bitwise(gpu, oracle, dtype, pattern, radius):
     assert abs(norm - oracle.l1norm_lp(dtype, n, radius, ref_out)) <= 1e-13 * abs(norm)
 
 
-@pytest.mark.parametrize("dtype,bound", [(F32, 2e-5), (BF16, 0.1)])
+# normwise error against FP64 of this case: FP32 2.23e-7, BF16 0.0145
+# (from the oracle's FP32/BF16 arithmetic, which the kernels match bit for
+# bit); the bounds are 1.5x those
+@pytest.mark.parametrize("dtype,bound", [(F32, 3.4e-7), (BF16, 0.022)])
 def test_storage_form_error_vs_f64(gpu, oracle, dtype, bound):
     n, iters, pattern, radius = 128, 10, 1, 2
     b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
