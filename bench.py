@@ -171,26 +171,49 @@ def solve_bytes(spec) -> int:
     return 2 * s * n * n + (iters + 1) * (2 * s * n * n + 2 * s * m * m) + s * m * m
 
 
+def class_sample(specs, per_class: int = 2):
+    """Indices of a sample of the solves: the first `per_class` solves of
+    each (pattern, radius, dtype) class, in submission order."""
+    seen, idx = {}, []
+    for i, s in enumerate(specs):
+        k = (s[0], s[1], s[2])
+        if seen.get(k, 0) < per_class:
+            seen[k] = seen.get(k, 0) + 1
+            idx.append(i)
+    return idx
+
+
 def batch_errors_vs_f64(Batch, K, specs, idx, outs) -> dict:
-    """Errors of FP32-storage results against the same solves run in FP64,
-    over the interior: max and mean absolute and relative (relative to
-    max(|y|, 1e-3 mean|y|)) and normwise L2, the worst over the sample."""
+    """Errors of reduced-precision results against the same solves run in
+    FP64, over the interior: max and mean absolute and relative (relative
+    to max(|y|, 1e-3 mean|y|)) and normwise L2.  Returns the worst over the
+    sample and the worst per (pattern, radius, storage) class."""
     import numpy as np
     b64 = Batch([(s[0], s[1], K.F64) + tuple(s[3:]) for s in (specs[i] for i in idx)])
     b64.run()
     b64.wait()
-    worst = {"max_abs": 0.0, "mean_abs": 0.0, "max_rel": 0.0, "mean_rel": 0.0, "normwise_l2": 0.0}
+    keys = ("max_abs", "mean_abs", "max_rel", "mean_rel", "normwise_l2")
+    worst = dict.fromkeys(keys, 0.0)
+    by_class = {}
     for j, i in enumerate(idx):
-        radius, n = specs[i][1], specs[i][3]
+        pattern, radius, dtype, n = specs[i][:4]
         inner = (slice(radius, n - radius),) * 2
         y = b64.output(j)[inner]
-        d = np.abs(outs[i].astype(np.float64)[inner] - y)
+        x = outs[i]
+        x = (x.astype(np.uint32) << 16).view(np.float32) if x.dtype == np.uint16 else x
+        d = np.abs(x.astype(np.float64)[inner] - y)
         rel = d / np.maximum(np.abs(y), 1e-3 * float(np.abs(y).mean()))
         e = {"max_abs": float(d.max()), "mean_abs": float(d.mean()), "max_rel": float(rel.max()),
              "mean_rel": float(rel.mean()), "normwise_l2": float(np.linalg.norm(d) / np.linalg.norm(y))}
-        worst = {k: max(worst[k], e[k]) for k in worst}
+        worst = {k: max(worst[k], e[k]) for k in keys}
+        name = f"{DT_NAME[dtype]} {'star' if pattern == 0 else 'grid'}{radius}"
+        c = by_class.setdefault(name, {"solves": 0, **dict.fromkeys(keys, 0.0)})
+        c["solves"] += 1
+        for k in keys:
+            c[k] = max(c[k], e[k])
     b64.close()
-    return {"solves": len(idx), **{k: float(f"{v:.3e}") for k, v in worst.items()}}
+    fmt = lambda d: {k: (float(f"{v:.3e}") if isinstance(v, float) else v) for k, v in d.items()}
+    return {"solves": len(idx), **fmt(worst), "by_class": {k: fmt(v) for k, v in sorted(by_class.items())}}
 
 
 def bench_batch(a) -> int:
@@ -226,8 +249,9 @@ def bench_batch(a) -> int:
     e1.record(stream)
     norms = b.wait()
     ms = e0.elapsed_time(e1)
-    # the FP32 results of a sample of the solves, against the same solves in FP64
-    sample_idx = list(range(0, len(specs), len(specs) // 16))[:16]
+    # the FP32 results of a sample of the solves (two of every class),
+    # against the same solves in FP64
+    sample_idx = class_sample(specs)
     sample_out = {i: b.output(i) for i in sample_idx}
     b.close()
     torch.cuda.empty_cache()
@@ -340,7 +364,7 @@ def bench_table(a) -> int:
         "cases": cases,
     }
     if peak:
-        res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
+        res["hbm_measured"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in peak.items()}
     print(json.dumps(res))
     return 0 if all(c["validated"] is not False for c in cases) else 1
 
@@ -385,14 +409,14 @@ def bench_transpose(a) -> int:
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (reference init A = 0..order^2-1, B = 0)",
         "config": {"workload": f"matrix transpose B += A^T, A += 1, order={order}", "order": order},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload)[0],
                      "algorithmic_bytes_per_launch": bytes_iter},
         "validated": abserr < 1e-8, "abserr": abserr,
         "rate_mb_s_reference_formula": round(1e-6 * bytes_iter / (ms * 1e-3), 1),
     }
     if peak:
-        res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
-        res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+        res["hbm_measured"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in peak.items()}
+        res["frac_of_measured_peak"] = round(gbs / max(v for v in peak.values() if isinstance(v, float)), 4)
         res["ratio_to_sweep_mix"] = round(gbs / peak["sweep_mix"], 4)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline_transpose(order)
@@ -495,7 +519,7 @@ def bench_slab(a) -> int:
                                f"plus {top}+{bot} halo rows), exchange not included",
                    "n": n, "slabs": parts, "row0": row0, "rows": rows, "halo_rows": [top, bot]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload)[0],
                      "algorithmic_bytes_per_launch": bytes_iter},
         "tuning": {**tuning.as_dict(), "autotuned": not a.no_tune},
         "validated": None,
@@ -529,16 +553,26 @@ def cpu_baseline_transpose(order: int):
             "avg_s_per_iter": avg}
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(workload: str, tiling=None):
     """HBM bytes per sweep launch measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE, scaled by the copy-kernel calibration (profiles/)."""
+    WRITE_SIZE, scaled by the copy-kernel calibration (profiles/).  A
+    profile lists one `traffic_bytes_per_launch[variant rows wgs order]:`
+    line per tiling it measured; the figure returned is the one of `tiling`
+    (variant, rows_per_wave, wgs_per_cu, order), or None when that tiling was
+    not profiled.  Returns (bytes or None, the profiled tilings)."""
     p = ROOT / "profiles" / f"sweep_{workload}.md"
     if not p.exists():
         p = ROOT / "profiles" / f"{workload}.md"
     if not p.exists():
-        return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text(), re.M)
-    return float(m.group(1)) if m else None
+        return None, []
+    text = p.read_text()
+    rows = {tuple(int(x) for x in m.group(1).split()): float(m.group(2))
+            for m in re.finditer(r"^traffic_bytes_per_launch\[([0-9 ]+)\]:\s*([0-9.eE+]+)", text, re.M)}
+    profiled = [" ".join(map(str, k)) for k in sorted(rows)]
+    if tiling is None:
+        m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+        return (float(m.group(1)) if m else None), profiled
+    return rows.get(tuple(tiling)), profiled
 
 
 def copy_calibration(torch, K, stream) -> None:
@@ -553,16 +587,73 @@ def copy_calibration(torch, K, stream) -> None:
     torch.cuda.empty_cache()
 
 
+# Copy shapes (16-byte vectors per lane, non-temporal, workgroups; 0 = one
+# per chunk) of the copy-shape sweep; the default line measures the
+# COPY_SHAPES_PEAK subset, the fastest of the sweep.
+COPY_SHAPES = [(v, nt, b) for v in (1, 2, 4, 8) for nt in (1, 0) for b in (0, 1024, 2048, 4096)]
+COPY_SHAPES_PEAK = [(1, 1, 0), (2, 1, 0), (4, 1, 0), (4, 0, 0), (2, 1, 2048), (4, 1, 2048)]
+
+
+def time_copy_shapes(torch, K, shapes, nbytes: int = 4 << 30, reps: int = 10) -> dict:
+    """GB/s (read + written bytes) of prk_copy_shape per shape, on nbytes."""
+    s = torch.cuda.current_stream()
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    out = {}
+    for vpl, nt, blocks in shapes:
+        def fn():
+            K.check(K.lib().prk_copy_shape(nbytes, dst.data_ptr(), src.data_ptr(), vpl, nt, blocks, s.cuda_stream))
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        out[f"vpl{vpl}_nt{nt}_wg{blocks}"] = 2 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    ok = bool(torch.equal(src, dst))
+    del src, dst
+    torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("copy shape produced a wrong copy")
+    return out
+
+
+def bench_copy_shapes(a) -> int:
+    """The copy probe in every shape of COPY_SHAPES on 4 GiB (10 copies
+    each): the fastest is the measured peak the stencil is held against."""
+    import torch
+    import kernels_amd as K
+
+    torch.cuda.set_device(0)
+    rates = time_copy_shapes(torch, K, COPY_SHAPES)
+    best = max(rates, key=rates.get)
+    res = {
+        "metric": METRIC, "value": round(rates[best], 1), "unit": "GB/s", "n_gpus": 1, "steps": 10,
+        "warmup": 1, "ms_per_step": round(2 * (4 << 30) / (rates[best] * 1e9) * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (4 GiB of ones)",
+        "config": {"workload": "copy probe, 4 GiB, every shape (vectors per lane, non-temporal, workgroups)"},
+        "best_shape": best, "peak_fraction_of_8tbs": round(rates[best] / HBM_PEAK_GBS, 4),
+        "shapes_gbs": {k: round(v, 1) for k, v in sorted(rates.items(), key=lambda kv: -kv[1])},
+    }
+    print(json.dumps(res))
+    return 0
+
+
 def measure_hbm_peak(torch, K, nbytes: int = 4 << 30, reps: int = 10):
     """Own copy, triad and sweep-mix kernels: the measured HBM peak on this
-    GPU, and the rate of a plain elementwise kernel with the sweep's mix of
-    two read and two write streams (a comparison, not a bound: the tiled
-    sweep moves its bytes a little faster than it)."""
+    GPU (the fastest of the copy shapes in COPY_SHAPES_PEAK, the fastest of
+    the full copy-shape sweep), and the rate of a plain elementwise kernel
+    with the sweep's mix of two read and two write streams (a comparison,
+    not a bound: the tiled sweep moves its bytes a little faster than it)."""
+    shapes = time_copy_shapes(torch, K, COPY_SHAPES_PEAK, nbytes, reps)
     s = torch.cuda.current_stream()
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     c = torch.empty_like(src).fill_(2.0)
-    res = {}
+    best = max(shapes, key=shapes.get)
+    res = {"copy_best": shapes[best], "copy_best_shape": best}
     for name, fn, bytes_per in (
             ("copy", lambda: K.check(K.lib().prk_copy(nbytes, dst.data_ptr(), src.data_ptr(), s.cuda_stream)),
              2 * nbytes),
@@ -612,6 +703,125 @@ def cpu_baseline(pattern: int, radius: int, n: int):
             "avg_s_per_iter": avg}
 
 
+def fp32_storage_section(a, torch, K, ms_f64: float) -> dict:
+    """The FP32-storage form of the headline stencil (FP32 star R=2,
+    n=30000): its points/s against the FP64 sweep just timed (analytic init,
+    tuned on its own grids, min(steps, 20) sweeps after one warm-up sweep,
+    validated to 1e-4), and its error against FP64 after 50 sweeps from
+    seeded U(-1,1) values (seed 42) over the whole interior: max and mean
+    absolute and relative (relative to max(|y|, 1e-3 mean|y|)) and the
+    normwise L2 error, beside the 1e-6 the FP32 form aims for."""
+    n, radius, pattern = 30000, 2, 0
+    stream = torch.cuda.current_stream()
+    steps = min(a.steps, 20)
+    bufs = [K.alloc_grid(n, K.F32), K.alloc_grid(n, K.F32)]
+    out = K.alloc_grid(n, K.F32)
+    if not a.no_tune:
+        K.autotune(pattern, radius, K.F32, bufs[0], bufs[1], out, 2, stream)
+    K.init(K.INIT_ANALYTIC, 0, K.F32, bufs[0], out, stream)
+    K.sweep(pattern, radius, K.F32, bufs[0], bufs[1], out, stream)
+    cur = 1
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        K.sweep(pattern, radius, K.F32, bufs[cur], bufs[cur ^ 1], out, stream)
+        cur ^= 1
+    e1.record(stream)
+    e1.synchronize()
+    ms32 = e0.elapsed_time(e1) / steps
+    norm = K.l1norm(K.F32, radius, out, stream)
+    validated = abs(norm - 2.0 * (steps + 1)) <= 1e-4
+    # seeded init, 50 sweeps after the first, FP32 then FP64
+    iters = 50
+    K.init(K.INIT_SEEDED, 42, K.F32, bufs[0], out, stream)
+    K.run(pattern, radius, K.F32, iters, bufs[0], bufs[1], out, stream)
+    del bufs
+    torch.cuda.empty_cache()
+    b64 = [K.alloc_grid(n, K.F64), K.alloc_grid(n, K.F64)]
+    y = K.alloc_grid(n, K.F64)
+    K.init(K.INIT_SEEDED, 42, K.F64, b64[0], y, stream)
+    K.run(pattern, radius, K.F64, iters, b64[0], b64[1], y, stream)
+    del b64
+    torch.cuda.empty_cache()
+    tot = {"sum_abs": 0.0, "sum_rel": 0.0, "max_abs": 0.0, "max_rel": 0.0, "d2": 0.0, "y2": 0.0}
+    inner = slice(radius, n - radius)
+    tau = 1e-3 * float(y[inner, inner].abs().mean())
+    for r0 in range(radius, n - radius, 2048):
+        r1 = min(r0 + 2048, n - radius)
+        yy = y[r0:r1, inner]
+        d = (out[r0:r1, inner].double() - yy).abs()
+        rel = d / torch.clamp(yy.abs(), min=tau)
+        tot["sum_abs"] += float(d.sum()); tot["sum_rel"] += float(rel.sum())
+        tot["max_abs"] = max(tot["max_abs"], float(d.max())); tot["max_rel"] = max(tot["max_rel"], float(rel.max()))
+        tot["d2"] += float((d * d).sum()); tot["y2"] += float((yy * yy).sum())
+    pts = (n - 2 * radius) ** 2
+    normwise = (tot["d2"] / tot["y2"]) ** 0.5
+    del out, y
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"star R=2 f32 storage (f32 accumulation) n={n} fused sweep",
+        "ms_per_step": round(ms32, 4), "steps": steps,
+        "gbs": round(K.stencil_bytes_per_iter(n, radius, K.F32) / (ms32 * 1e-3) / 1e9, 1),
+        "points_per_s": round((n - 2 * radius) ** 2 / (ms32 * 1e-3), 0),
+        "points_per_s_vs_f64": round(ms_f64 / ms32, 3), "target_points_per_s_vs_f64": 1.8,
+        "validated": bool(validated),
+        "error_vs_f64_seeded": {"iterations": iters, "seed": 42, "normwise_l2": float(f"{normwise:.3e}"),
+                                "max_abs": float(f"{tot['max_abs']:.3e}"),
+                                "mean_abs": float(f"{tot['sum_abs'] / pts:.3e}"),
+                                "max_rel": float(f"{tot['max_rel']:.3e}"),
+                                "mean_rel": float(f"{tot['sum_rel'] / pts:.3e}"),
+                                "target_normwise": 1e-6, "target_met": bool(normwise <= 1e-6)},
+    }
+
+
+def batch_section(a, torch, K) -> dict:
+    """The 4096 uneven grids of the batched workload (FP32 storage) in one
+    hipGraph of round launches, against the same solves one grid per launch
+    in FP64: solves/s of each (after tuning the tile heights unless
+    --no-tune; one warm-up run, then three timed runs of the batch)."""
+    import numpy as np
+    from kernels_amd.batch import Batch, autotune, one_by_one
+
+    specs = config4_solves()
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    if not a.no_tune:
+        autotune(specs, sample=16, reps=2)
+    t_tune = time.perf_counter() - t0
+    b = Batch(specs)
+    b.run(stream.cuda_stream)
+    b.wait()
+    runs = 3
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(runs):
+        b.run(stream.cuda_stream)
+    e1.record(stream)
+    norms = b.wait()
+    ms = e0.elapsed_time(e1) / runs
+    sample_idx = class_sample(specs)
+    sample_out = {i: b.output(i) for i in sample_idx}
+    b.close()
+    torch.cuda.empty_cache()
+    errors = batch_errors_vs_f64(Batch, K, specs, sample_idx, sample_out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n64 = one_by_one(specs, dtype_override=K.F64)
+    t64 = time.perf_counter() - t0
+    solves_s = len(specs) / (ms * 1e-3)
+    return {
+        "workload": "4096 grids (seed 7): n log-uniform [64,2048], star/grid, R 1..4, iterations 1..50, "
+                    "FP32 storage, one hipGraph of round launches",
+        "solves_per_s": round(solves_s, 1), "ms_per_run": round(ms, 2), "runs": runs,
+        "effective_gbs": round(sum(solve_bytes(s) for s in specs) / (ms * 1e-3) / 1e9, 1),
+        "one_by_one_f64_solves_per_s": round(len(specs) / t64, 1),
+        "speedup_vs_one_by_one_f64": round(solves_s * t64 / len(specs), 2), "target_speedup": 5.0,
+        "max_rel_norm_diff_vs_f64": float(f"{float(np.max(np.abs(norms - n64) / np.abs(n64))):.3e}"),
+        "fp32_vs_fp64_sample": errors,
+        "tile_tune_s": round(t_tune, 2),
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -619,7 +829,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="star2_f64_n30000",
                     choices=sorted(WORKLOADS) + list(BATCH_WORKLOADS) + list(SERVE_WORKLOADS) + list(TABLE_WORKLOADS)
-                            + list(TRANSPOSE_WORKLOADS) + list(SLAB_WORKLOADS))
+                            + list(TRANSPOSE_WORKLOADS) + list(SLAB_WORKLOADS) + ["copy_shapes"])
     ap.add_argument("--serve-rates", type=float, nargs="+", default=list(SERVE_RATES))
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -634,6 +844,8 @@ def main(argv=None) -> int:
                     help="slab workloads: skip the whole-grid comparison sweeps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the sweeps, run one 2 GiB copy (known bytes) for counter calibration")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload: skip the FP32-storage and batched (4096 grids) sections")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): the halo exchange between GPUs is "
@@ -652,6 +864,8 @@ def main(argv=None) -> int:
         return bench_transpose(a)
     if a.workload in SLAB_WORKLOADS:
         return bench_slab(a)
+    if a.workload == "copy_shapes":
+        return bench_copy_shapes(a)
     if a.workload in SERVE_WORKLOADS:
         # The first run of this open-loop serving benchmark on MI355X (500
         # req/s for 10 s, 200 GiB pool) ended in a host segmentation fault
@@ -728,6 +942,15 @@ def main(argv=None) -> int:
     bytes_iter = K.stencil_bytes_per_iter(n, radius, dtype)
     ms_step = ms / a.steps
     gbs = bytes_iter / (ms_step * 1e-3) / 1e9
+    # the tiling that ran (tuned, or the table's entry with --no-tune)
+    used = K.Tuning()
+    K.check(K.lib().prk_tuning_get(pattern, radius, dtype, n, used))
+    tiling_key = (used.variant, used.rows_per_wave, used.wgs_per_cu, used.order)
+    traffic, profiled = traffic_from_profiles(a.workload, tiling_key)
+    extra = {}
+    if not a.no_extra and a.workload == "star2_f64_n30000":
+        extra["fp32_storage"] = fp32_storage_section(a, torch, K, ms_step)
+        extra["batched_config4"] = batch_section(a, torch, K)
     peak = {} if a.no_peak else measure_hbm_peak(torch, K)
     res = {
         "metric": METRIC,
@@ -747,7 +970,9 @@ def main(argv=None) -> int:
                    "pattern": "star" if pattern == 0 else "grid", "radius": radius, "n": n,
                    "iterations": a.steps, "tile": 32},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(a.workload),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_tiling": " ".join(map(str, tiling_key)),
+                     "traffic_profiled_tilings": profiled,
                      "algorithmic_bytes_per_launch": bytes_iter},
         "tuning": {**tuning.as_dict(), "autotuned": not a.no_tune, "tune_s": round(t_tune, 2)},
         "validated": validated if validated is None else bool(validated),
@@ -760,13 +985,15 @@ def main(argv=None) -> int:
                     "gbs_median": round(bytes_iter / (sorted(rep_ms)[2] * 1e-3) / 1e9, 1)},
     }
     if peak:
-        res["hbm_measured"] = {k: round(v, 1) for k, v in peak.items()}
-        res["frac_of_measured_peak"] = round(gbs / max(peak.values()), 4)
+        res["hbm_measured"] = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in peak.items()}
+        res["frac_of_measured_peak"] = round(gbs / max(v for v in peak.values() if isinstance(v, float)), 4)
         res["ratio_to_sweep_mix"] = round(gbs / peak["sweep_mix"], 4)
+    res.update(extra)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(pattern, radius, n) if dtype == 0 else None
     print(json.dumps(res))
-    return 1 if validated is False else 0
+    bad = validated is False or extra.get("fp32_storage", {}).get("validated") is False
+    return 1 if bad else 0
 
 
 if __name__ == "__main__":

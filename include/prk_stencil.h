@@ -168,7 +168,9 @@ int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iter
 /* Tiling overrides for the autotuner and tests (process-wide; they never
  * change results).  rows: row-block height per wavefront, 0 = automatic.
  * variant: fused-sweep pipeline variant 0..5 (pipeline depth 1..3 x
- * temporal / non-temporal streams), -1 = the tuned or default choice. */
+ * temporal / non-temporal streams), 6 and 7 the register-window form of the
+ * FP32/BF16 sweeps at R <= 3 (depth 2 and 3, non-temporal; other shapes run
+ * them as 4 and 5), -1 = the tuned or default choice. */
 int prk_set_rows_per_wave(int rows);
 int prk_set_sweep_variant(int variant);
 /* order: workgroup-to-tile order (0 column-major runs per XCD, 1 row-major,
@@ -348,6 +350,15 @@ int prk_copy(long long bytes, void *dst, const void *src, void *stream);
  * sweep moves its bytes about 4 % faster than this.  No reference
  * counterpart; a calibration in the manner of the nstream triad above. */
 int prk_stream_mix(long long len, const double *in, double *in_next, double *out, void *stream);
+/* dst = src (bytes, a multiple of 16) in a chosen shape, to find the
+ * fastest streaming pattern on the device: each lane moves `vpl` 16-byte
+ * vectors (1, 2, 4 or 8; a workgroup of 256 lanes covers 256*vpl vectors,
+ * lanes adjacent within each vector row, every load issued before the first
+ * store); `nt` 1 marks loads and stores non-temporal; `blocks` 0 launches
+ * one workgroup per 256*vpl vectors, > 0 launches that many workgroups that
+ * stride over the array (persistent form).  prk_copy is vpl 1, nt 1,
+ * blocks 0.  No reference counterpart. */
+int prk_copy_shape(long long bytes, void *dst, const void *src, int vpl, int nt, int blocks, void *stream);
 
 /* ------------------------------------------------------------------------
  * Matrix transpose, the next bandwidth-bound kernel of the suite.  One

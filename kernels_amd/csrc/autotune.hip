@@ -1,6 +1,7 @@
 // Autotuner for the fused sweep: for a (pattern, radius, dtype, n) it times
 // every candidate tiling on the device -- pipeline variant (depth 1..3 rows,
-// temporal or non-temporal streams), row-block height per wavefront and
+// temporal or non-temporal streams, and for the FP32/BF16 forms at R <= 3
+// the register-window sweep), row-block height per wavefront and
 // resident workgroups per CU -- and records the fastest in a process-wide
 // table that plan_sweep() consults.  The table can be saved to and loaded
 // from a text file.  Tiling never changes results, only speed.
@@ -25,7 +26,7 @@ typedef std::tuple<int, int, int, int> Key;  // pattern, radius, dtype, n
 std::mutex g_mu;
 std::map<Key, prk_tuning> g_table;
 
-const int kCandVariants[] = {1, 2, 4, 5};
+const int kCandVariants[] = {1, 2, 4, 5, 6, 7};
 const int kCandWgs[] = {2, 3, 4};
 const int kCandRows[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
 // Row blocks beyond 32 only for R >= 4, where the 2R rows each block re-reads
@@ -187,8 +188,10 @@ int autotune_on(int pattern, int radius, int dtype, int n, long long pitch, void
     // Stage 1: every candidate, `reps` sweeps each.
     std::vector<prk_tuning> cands;
     for (int v : kCandVariants) {
-        // beyond the reference's radii one variant is built (see launch_pr)
+        // beyond the reference's radii one variant is built (see launch_pr);
+        // 6 and 7 exist only where the register-window sweep applies
         if (radius > kRefMaxRadius && v != 4) continue;
+        if (v >= 6 && !register_window_variant(v, radius, dtype)) continue;
         for (int w : kCandWgs) {
             for (int rows : kCandRows) {
                 if (rc != PRK_OK) break;

@@ -353,6 +353,38 @@ __global__ __launch_bounds__(256) void copy_kernel(long long nvec, u32x4 *dst, c
     if (k < nvec) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[k]), &dst[k]);
 }
 
+// Copy in a chosen shape: a workgroup's chunk is VPL rows of 256 vectors
+// (lane t takes vector t of each row), all VPL loads issued before the
+// stores; chunks are taken in turn by `gridDim.x` workgroups.
+template <int VPL, bool NT>
+__global__ __launch_bounds__(256) void copy_shape_kernel(long long nvec, u32x4 *dst, const u32x4 *src)
+{
+    const long long chunk = 256LL * VPL;
+    for (long long base = (long long)blockIdx.x * chunk; base < nvec; base += (long long)gridDim.x * chunk) {
+        u32x4 v[VPL];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const long long k = base + j * 256 + threadIdx.x;
+            if (k < nvec) v[j] = NT ? __builtin_nontemporal_load(&src[k]) : src[k];
+        }
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const long long k = base + j * 256 + threadIdx.x;
+            if (k < nvec) {
+                if (NT) __builtin_nontemporal_store(v[j], &dst[k]);
+                else dst[k] = v[j];
+            }
+        }
+    }
+}
+
+template <int VPL>
+static void launch_copy_shape(int nt, unsigned blocks, long long nvec, u32x4 *dst, const u32x4 *src, hipStream_t s)
+{
+    if (nt) hipLaunchKernelGGL((copy_shape_kernel<VPL, true>), dim3(blocks), dim3(256), 0, s, nvec, dst, src);
+    else hipLaunchKernelGGL((copy_shape_kernel<VPL, false>), dim3(blocks), dim3(256), 0, s, nvec, dst, src);
+}
+
 // ---------------------------------------------------------------------------
 // The norm's partials in a static device array, for the allocation-free
 // entry point.  In host code `scratch` names a host-side stand-in, not device
@@ -612,6 +644,27 @@ int prk_stream_mix(long long len, const double *in, double *in_next, double *out
     const unsigned blocks = (unsigned)std::max<long long>(1, (nv + 255) / 256);
     hipLaunchKernelGGL(stream_mix_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, len, in, in_next, out);
     return check_hip(hipGetLastError(), "stream mix launch");
+}
+
+int prk_copy_shape(long long bytes, void *dst, const void *src, int vpl, int nt, int blocks, void *stream)
+{
+    if (bytes < 16 || bytes % 16 || !dst || !src || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return set_error(PRK_EINVAL, "copy needs a positive multiple of 16 bytes, 16-byte aligned");
+    if (vpl != 1 && vpl != 2 && vpl != 4 && vpl != 8) return set_error(PRK_EINVAL, "vpl must be 1, 2, 4 or 8");
+    if (nt != 0 && nt != 1) return set_error(PRK_EINVAL, "nt must be 0 or 1");
+    if (blocks < 0) return set_error(PRK_EINVAL, "blocks must be >= 0");
+    const long long nvec = bytes / 16;
+    const long long chunks = (nvec + 256LL * vpl - 1) / (256LL * vpl);
+    if (chunks >= (1LL << 31)) return set_error(PRK_EINVAL, "copy too large");
+    const unsigned grid = blocks > 0 ? (unsigned)std::min<long long>(blocks, chunks) : (unsigned)chunks;
+    hipStream_t s = (hipStream_t)stream;
+    switch (vpl) {
+    case 1: launch_copy_shape<1>(nt, grid, nvec, (u32x4 *)dst, (const u32x4 *)src, s); break;
+    case 2: launch_copy_shape<2>(nt, grid, nvec, (u32x4 *)dst, (const u32x4 *)src, s); break;
+    case 4: launch_copy_shape<4>(nt, grid, nvec, (u32x4 *)dst, (const u32x4 *)src, s); break;
+    default: launch_copy_shape<8>(nt, grid, nvec, (u32x4 *)dst, (const u32x4 *)src, s); break;
+    }
+    return check_hip(hipGetLastError(), "copy launch");
 }
 
 int prk_copy(long long bytes, void *dst, const void *src, void *stream)

@@ -28,8 +28,15 @@ struct SweepPlan {
     int wgs_per_cu;     // occupancy cap, 0 = none
 };
 
-// Number of fused-sweep variants (pipeline depth 1..3 x temporal/non-temporal).
-constexpr int kSweepVariants = 6;
+// Number of fused-sweep variants: 0..5 the LDS-ring sweep (pipeline depth
+// 1..3 x temporal / non-temporal), 6 and 7 the register-window sweep of the
+// FP32/BF16 forms at R <= 3 (depth 2 and 3, non-temporal; elsewhere they run
+// as 4 and 5).
+constexpr int kSweepVariants = 8;
+inline bool register_window_variant(int variant, int radius, int dtype)
+{
+    return (variant == 6 || variant == 7) && dtype != F64 && radius <= 3;
+}
 
 // The L1 norm's first pass leaves one double-double partial per block, at
 // most kNormBlocks of them, in a device buffer of kNormPartialsBytes.

@@ -601,10 +601,22 @@ enum { MODE_ELEM = 0, MODE_VEC = 1, MODE_VEC_LD = 2 };
 // element LDS ring.  LDAUX adds cache policy bits to every HBM load (the
 // batch kernel bypasses L1 with it, since neighbouring tiles written by
 // other CUs are read there).
-template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX>
-__device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                 const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                 typename DTypeTraits<DT>::T *ring_base, int lane)
+// How a wave stores out (whole-vector mode).  In the first and last strips
+// of a row a lane's vector may mix interior and frame columns:
+//   FRAME_KEEP     such lanes store their interior elements one by one
+//                  (frame elements out of range): the frame is never
+//                  written, as the reference's operators promise;
+//   FRAME_REWRITE  every lane stores the whole vector with the frame
+//                  elements as they were loaded (grids the library owns,
+//                  where nothing else writes the frame: one store instead
+//                  of V, and no edge copy of the tile code).
+// FRAME_INTERIOR waves (every other strip) hold interior columns only.
+enum { FRAME_INTERIOR = 0, FRAME_KEEP = 1, FRAME_REWRITE = 2 };
+
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX, int FRAME>
+__device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                      typename DTypeTraits<DT>::T *ring_base, int lane)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
@@ -634,10 +646,17 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
     const int lh = lane < NH ? lane * V : HP + SW + (lane - NH) * V;  // LDS column
     const unsigned off_main = (unsigned)(cm * ES);
     const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;  // ch<0 wraps out of range
-    // Per-column write-back mask for out: frame columns keep their value.
-    bool interior[V];
+    // Per-column store offsets of out (EDGE waves, and element modes): frame
+    // columns go out of range and are never written.
+    unsigned off_out[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) interior[v] = (cm + v >= R) && (cm + v < n - R);
+    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
+    // EDGE waves: lanes whose whole vector is interior still store it whole;
+    // the others store their interior elements one by one
+    const bool vec_interior = cm >= R && cm + V <= n - R;
+    unsigned off_part[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
 
     T(*ring)[LR] = (T(*)[LR])ring_base;
 
@@ -717,18 +736,50 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
                 }
                 union { u32x4 v; T e[V]; } res;
                 compute_points<DT, P, R, V, HP>(rows, cur.outv.e, res.e);
-#pragma unroll
-                for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : cur.outv.e[v];
                 const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
-                if constexpr (MODE == MODE_VEC) {
+                if constexpr (MODE == MODE_VEC && FRAME == FRAME_INTERIOR) {
                     bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else if constexpr (MODE == MODE_VEC && FRAME == FRAME_REWRITE) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) res.e[v] = off_out[v] != kOOB ? res.e[v] : cur.outv.e[v];
+                    bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
                 } else {
 #pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_main + v * ES, res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
                 }
             }
             slot = (slot + 1 == S) ? 0 : slot + 1;
         }
+    }
+}
+
+// KEEP_FRAME false: the grid belongs to the library (batched and served
+// solves), so every wave stores whole vectors, the frame elements of out as
+// they were loaded (FRAME_REWRITE), from one copy of the tile code: the
+// batch runs dozens of these kernels side by side, and with an edge and an
+// interior copy of every tile loop it ran 8 % slower (same box, alternating
+// runs).
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX, bool KEEP_FRAME = true>
+__device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                 const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                 typename DTypeTraits<DT>::T *ring_base, int lane)
+{
+    if constexpr (!KEEP_FRAME) {
+        wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_REWRITE>(in_, in_next_, out_, pitch, n, sp, c0,
+                                                                           r0, r1, pad_writable, ring_base, lane);
+    } else {
+        // wave-uniform: only the first and last strips of a row hold frame columns
+        if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
+            wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_KEEP>(in_, in_next_, out_, pitch, n, sp, c0,
+                                                                            r0, r1, pad_writable, ring_base, lane);
+        else
+            wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_INTERIOR>(in_, in_next_, out_, pitch, n, sp,
+                                                                                c0, r0, r1, pad_writable,
+                                                                                ring_base, lane);
     }
 }
 
@@ -737,10 +788,10 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
 // slots (M >= 2R+1+PF, a multiple of PF; row ri in slot (ri - lo) % M; the
 // row loop is unrolled by M so every slot index is a constant) and an LDS
 // ring of R+1 rows for the centre rows.
-template <int DT, int R, bool FUSE, int MODE, int PF, int NT>
-__device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                      typename DTypeTraits<DT>::T *ring_base, int lane)
+template <int DT, int R, bool FUSE, int MODE, int PF, int NT, bool EDGE>
+__device__ inline void wave_tile_star_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                           const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                           typename DTypeTraits<DT>::T *ring_base, int lane)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
@@ -765,9 +816,15 @@ __device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out
     const int lh = lane < NH ? lane * V : HP + G::SW + (lane - NH) * V;
     const unsigned off_main = (unsigned)(cm * ES);
     const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;
-    bool interior[V];
+    unsigned off_out[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) interior[v] = (cm + v >= R) && (cm + v < n - R);
+    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
+    // EDGE waves: lanes whose whole vector is interior still store it whole;
+    // the others store their interior elements one by one
+    const bool vec_interior = cm >= R && cm + V <= n - R;
+    unsigned off_part[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
 
     T(*ring)[LR] = (T(*)[LR])ring_base;
 
@@ -847,14 +904,16 @@ __device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out
                     for (int v = 0; v < V; ++v) col[j][v] = qm[(k + M - 2 * R + j) % M].e[v];
                 Vec res;
                 star_points<DT, R, V, HP>(&ring[cs][HP + lane * V], col, sd.outv.e, res.e);
-#pragma unroll
-                for (int v = 0; v < V; ++v) res.e[v] = interior[v] ? res.e[v] : sd.outv.e[v];
                 const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
-                if constexpr (MODE == MODE_VEC) {
+                if constexpr (MODE == MODE_VEC && !EDGE) {
                     bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
                 } else {
 #pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_main + v * ES, res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
                 }
             }
             lslot = lslot + 1 == L ? 0 : lslot + 1;
@@ -862,6 +921,270 @@ __device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out
     }
 }
 
+template <int DT, int R, bool FUSE, int MODE, int PF, int NT>
+__device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                      typename DTypeTraits<DT>::T *ring_base, int lane)
+{
+    if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
+        wave_tile_star_impl<DT, R, FUSE, MODE, PF, NT, true>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
+                                                             pad_writable, ring_base, lane);
+    else
+        wave_tile_star_impl<DT, R, FUSE, MODE, PF, NT, false>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
+                                                              pad_writable, ring_base, lane);
+}
+
+
+// ---------------------------------------------------------------------------
+// Reduced-precision sweeps at small R from a register ring of widened row
+// windows.  The LDS-ring sweep widens every value it reads, so each input
+// row is widened once for every one of the 2R+1 output rows that read it,
+// and the partly used edge vectors of its windows are read from LDS in
+// pieces that conflict in the banks.  Here each input row is widened once,
+// when it arrives: the lane's own V columns from its load, the R columns on
+// either side from the neighbouring lanes' loads by whole-wave DPP shifts
+// (lane 0 and lane 63 take them from the strip's halo vectors, which lanes 0
+// and 1 load).  The windows of the last 2R+1 rows stay in registers (the row
+// loop is unrolled by 2R+1, so every slot is a constant register); no LDS is
+// used.  The arithmetic is that of compute_points, operand for operand, so
+// results are bitwise those of the LDS-ring sweep.
+// ---------------------------------------------------------------------------
+template <int DT, int P, int R>
+constexpr bool reg_window_ok() { return DT != F64 && R <= 3; }
+
+// Whole-wave lane shifts: lane i receives lane i-1's value (from_left) or
+// lane i+1's (from_right); lane 0, resp. lane 63, has no source and keeps
+// `edge`.
+__device__ inline unsigned from_left(unsigned x, unsigned edge)
+{
+    return (unsigned)__builtin_amdgcn_update_dpp((int)edge, (int)x, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+__device__ inline unsigned from_right(unsigned x, unsigned edge)
+{
+    return (unsigned)__builtin_amdgcn_update_dpp((int)edge, (int)x, 0x130 /* wave_shl:1 */, 0xF, 0xF, false);
+}
+
+// The window of one input row as loaded: the lane's own vector and the
+// dwords of its neighbours' vectors that hold the R columns on either side.
+// main is the lane's own vector; lhalo the vector of columns -V..-1 (used by
+// lane 0: the strip's left halo vector); rhalo the vector of columns
+// 64V..65V-1 of the strip (used by lane 63: the strip's right halo vector).
+template <typename T, int V, int R>
+struct PackedWindow {
+    static_assert(R <= V, "neighbour columns come from the adjacent lane only");
+    static constexpr int E = 4 / (int)sizeof(T);  // elements per dword
+    static constexpr int LD0 = (V - R) / E;       // first dword of the left neighbour used
+    static constexpr int NL = V / E - LD0;        // left neighbour dwords
+    static constexpr int NR = (R - 1) / E + 1;    // right neighbour dwords
+    u32x4 m;
+    unsigned l[NL], r[NR];
+
+    __device__ inline void put(const u32x4 &main, const u32x4 &lhalo, const u32x4 &rhalo)
+    {
+        m = main;
+#pragma unroll
+        for (int d = 0; d < NL; ++d) l[d] = from_left(main[LD0 + d], lhalo[LD0 + d]);
+#pragma unroll
+        for (int d = 0; d < NR; ++d) r[d] = from_right(main[d], rhalo[d]);
+    }
+    // w[i] = column i - R from the lane's first column, i in [0, V + 2R)
+    __device__ inline void widen(float (&w)[V + 2 * R]) const
+    {
+        union { u32x4 v; T e[V]; } mm, ll, rr;
+        mm.v = m;
+        ll.v = u32x4{0u, 0u, 0u, 0u};
+        rr.v = u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int d = 0; d < NL; ++d) ll.v[LD0 + d] = l[d];
+#pragma unroll
+        for (int d = 0; d < NR; ++d) rr.v[d] = r[d];
+#pragma unroll
+        for (int i = 0; i < R; ++i) w[i] = to_f32<T>(ll.e[V - R + i]);
+#pragma unroll
+        for (int i = 0; i < V; ++i) w[R + i] = to_f32<T>(mm.e[i]);
+#pragma unroll
+        for (int i = 0; i < R; ++i) w[R + V + i] = to_f32<T>(rr.e[i]);
+    }
+};
+
+// A ring slot: the window widened once, when its row arrives.  (Keeping the
+// BF16 windows as loaded and widening them where they are used does not
+// shrink the ring: the compiler keeps each slot's widened values live across
+// the unrolled steps, or, forced to widen at every use, still needs over 190
+// VGPRs at grid R=3.)
+template <typename T, int V, int R>
+struct RingSlot {
+    float w[V + 2 * R];
+    __device__ inline void put(const u32x4 &main, const u32x4 &lhalo, const u32x4 &rhalo)
+    {
+        PackedWindow<T, V, R> pw;
+        pw.put(main, lhalo, rhalo);
+        pw.widen(w);
+    }
+};
+
+// Paired sums over the ring: the window of row o + a is in slot
+// (k + 1 + R + a) % S for the output row of ring step k (rows o-R..o+R sit
+// in slots k+1..k+S).  k is the step of the fully unrolled row loop, so every
+// index is a constant once the loop is unrolled and the ring stays in
+// registers.
+template <int P, int R, int V, int S, int A, typename Slot>
+__device__ inline void rw_pair_sum(const Slot (&W)[S], int k, float (&acc)[V])
+{
+    constexpr int b = pair_begin<P, R>(A), e = pair_begin<P, R>(A + 1);
+    if constexpr (e > b)
+        pair_range<P, R, V, R, b>(W[(k + 1 + R + A) % S].w, W[(k + 1 + R - A) % S].w, acc,
+                                  std::make_index_sequence<e - b>{});
+    if constexpr (A < R) rw_pair_sum<P, R, V, S, A + 1>(W, k, acc);
+}
+
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, bool EDGE>
+__device__ inline void wave_tile_rw_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                         const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane)
+{
+    typedef SweepGeom<DT, R> G;
+    typedef typename G::T T;
+    constexpr int V = G::V, HP = G::HP, NH = G::NH, S = G::S;
+    constexpr int ES = (int)sizeof(T);
+    constexpr int AUX_STREAM = (NT & 1) ? 2 : 0;
+    static_assert(HP == V && NH == 1, "one halo vector per side");
+    static_assert(PF <= S, "rows in flight fit the ring");
+
+    const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
+    const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
+
+    const T *in = (const T *)in_;
+    T *in_next = (T *)in_next_;
+    T *out = (T *)out_;
+    const unsigned ld_bytes = (unsigned)(pitch * ES);
+    const unsigned st_bytes = pad_writable ? ld_bytes : (unsigned)(n * ES);
+
+    const int cm = c0 + lane * V;
+    // halo vectors: lane 0 the V columns left of the strip, lane 1 the V
+    // columns right of it; other lanes load nothing
+    const int ch = lane == 0 ? c0 - V : c0 + G::SW;
+    const unsigned off_main = (unsigned)(cm * ES);
+    const unsigned off_halo = lane < 2 ? (unsigned)(ch * ES) : kOOB;  // ch < 0 wraps out of range
+    unsigned off_out[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
+    // EDGE waves: lanes whose whole vector is interior still store it whole;
+    // the others store their interior elements one by one
+    const bool vec_interior = cm >= R && cm + V <= n - R;
+    unsigned off_part[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
+
+    // Row ri's loads (main, halo, out row ri - R) in raw[(ri - lo) % S]; its
+    // window in W[(ri - lo) % S].
+    RowRegs<T, V> raw[S];
+    RingSlot<T, V, R> W[S];
+    auto issue = [&](RowRegs<T, V> &r, int ri) {
+        const bool row_ok = ri < hi;
+        const int rr = row_ok ? ri : lo;
+        const __amdgpu_buffer_rsrc_t rin = row_rsrc(in + (long long)rr * pitch, ld_bytes);
+        const int o = ri - R;
+        const bool out_ok = o >= olo && o < ohi;
+        const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)(out_ok ? o : olo) * pitch, ld_bytes);
+        if constexpr (MODE != MODE_ELEM) {
+            r.main.v = bload16<0>(rin, row_ok ? off_main : kOOB);
+            r.halo.v = bload16<0>(rin, row_ok ? off_halo : kOOB);
+            r.outv.v = bload16<AUX_STREAM>(rout, out_ok ? off_main : kOOB);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                r.main.e[v] = bload_elem<T, 0>(rin, row_ok ? off_main + v * ES : kOOB);
+                r.halo.e[v] = bload_elem<T, 0>(rin, (row_ok && off_halo != kOOB) ? off_halo + v * ES : kOOB);
+                r.outv.e[v] = bload_elem<T, AUX_STREAM>(rout, out_ok ? off_main + v * ES : kOOB);
+            }
+        }
+    };
+
+#pragma unroll
+    for (int k = 0; k < PF; ++k) issue(raw[k], lo + k);
+
+    for (int base = lo; base < hi; base += S) {
+#pragma clang loop unroll(full)
+        for (int k = 0; k < S; ++k) {
+            const int ri = base + k;
+            if (ri >= hi) return;  // uniform: the same row in every lane
+            const RowRegs<T, V> cur = raw[k];
+            issue(raw[(k + PF) % S], ri + PF);
+
+            if constexpr (FUSE) {
+                const bool nx_ok = ri >= ulo && ri < uhi;
+                const __amdgpu_buffer_rsrc_t rnx =
+                    row_rsrc(in_next + (long long)(nx_ok ? ri : r0) * pitch, st_bytes);
+                union { u32x4 v; T e[V]; } nx;
+                plus_one<DT, V>(cur.main.e, nx.e);
+                if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rnx, nx_ok ? off_main : kOOB, nx.v);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        bstore_elem<T, AUX_STREAM>(rnx, nx_ok ? off_main + v * ES : kOOB, nx.e[v]);
+                }
+            }
+
+            // lane 63's right neighbours: the right halo vector, loaded by lane 1
+            u32x4 rhalo;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) rhalo[d] = (unsigned)__builtin_amdgcn_readlane((int)cur.halo.v[d], 1);
+            W[k].put(cur.main.v, cur.halo.v, rhalo);
+
+            const int o = ri - R;
+            if (o >= olo && o < ohi) {
+                float acc[V];
+                rw_pair_sum<P, R, V, S, 0>(W, k, acc);
+                float f[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) f[v] = to_f32<T>(cur.outv.e[v]) + acc[v];
+                union { u32x4 v; T e[V]; } res;
+                from_f32_vec<V>(f, res.e);
+                const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
+                if constexpr (MODE == MODE_VEC && !EDGE) {
+                    bstore16<AUX_STREAM>(rout, off_main, res.v);
+                } else if constexpr (MODE == MODE_VEC) {
+                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
+                }
+            }
+        }
+    }
+}
+
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+__device__ inline void wave_tile_rw(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                    const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane)
+{
+    if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
+        wave_tile_rw_impl<DT, P, R, FUSE, MODE, PF, NT, true>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
+                                                              pad_writable, lane);
+    else
+        wave_tile_rw_impl<DT, P, R, FUSE, MODE, PF, NT, false>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
+                                                               pad_writable, lane);
+}
+
+// The register-window sweep kernel: the launch geometry of sweep_kernel, no LDS.
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+__global__ __launch_bounds__(kThreads) void sweep_rw_kernel(SweepArgs a)
+{
+    int group, rb;
+    if (!tile_of(a, blockIdx.x, group, rb)) return;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int strip = group * kWavesPerBlock + wave;
+    if (strip >= a.nstrips) return;
+    const int r0 = rb * a.rows_per_wave;
+    const int r1 = min(r0 + a.rows_per_wave, a.span.nrows);
+    wave_tile_rw<DT, P, R, FUSE, MODE, PF, NT>(a.in, a.in_next, a.out, a.pitch, a.n, a.span,
+                                               strip * SweepGeom<DT, R>::SW, r0, r1, a.pad_writable, lane);
+}
 
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
 __global__ __launch_bounds__(kThreads)

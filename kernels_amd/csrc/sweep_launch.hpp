@@ -6,7 +6,7 @@
 
 namespace prk {
 
-template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, bool RW = false>
 int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hipStream_t s)
 {
     typedef SweepGeom<DT, R> G;
@@ -40,9 +40,12 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
     }
     // Extra dynamic LDS caps the workgroups resident per CU (occupancy knob).
     const size_t lds_cap = p.wgs_per_cu > 0 ? (160u * 1024u) / (unsigned)p.wgs_per_cu : 0;
-    constexpr size_t lds_bytes = sweep_lds_bytes<DT, P, R>();
+    constexpr size_t lds_bytes = RW ? 0 : sweep_lds_bytes<DT, P, R>();
     const size_t dyn = lds_cap > lds_bytes ? lds_cap - lds_bytes : 0;
-    hipLaunchKernelGGL((sweep_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
+    if constexpr (RW)
+        hipLaunchKernelGGL((sweep_rw_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
+    else
+        hipLaunchKernelGGL((sweep_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
     return check_hip(hipGetLastError(), "sweep launch");
 }
 
@@ -60,12 +63,16 @@ int launch_pr(const SweepPlan &p, const void *in, void *in_next, void *out, hipS
         if constexpr (R > kRefMaxRadius) {
             return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
         } else {
+            if constexpr (reg_window_ok<DT, P, R>()) {
+                if (v == 6) return launch_one<DT, P, R, true, MODE_VEC, 2, 1, true>(p, in, in_next, out, s);
+                if (v == 7) return launch_one<DT, P, R, true, MODE_VEC, 3, 1, true>(p, in, in_next, out, s);
+            }
             switch (v) {
             case 0: return launch_one<DT, P, R, true, MODE_VEC, 1, 0>(p, in, in_next, out, s);
             case 1: return launch_one<DT, P, R, true, MODE_VEC, 2, 0>(p, in, in_next, out, s);
             case 2: return launch_one<DT, P, R, true, MODE_VEC, 3, 0>(p, in, in_next, out, s);
             case 3: return launch_one<DT, P, R, true, MODE_VEC, 1, 1>(p, in, in_next, out, s);
-            case 4: return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
+            case 4: case 6: return launch_one<DT, P, R, true, MODE_VEC, 2, 1>(p, in, in_next, out, s);
             default: return launch_one<DT, P, R, true, MODE_VEC, 3, 1>(p, in, in_next, out, s);
             }
         }

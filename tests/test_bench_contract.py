@@ -44,10 +44,20 @@ def test_default_workload_line(gpu):
     roof = d["roofline"]
     assert roof["bound"] == "hbm" and roof["peak"] == 8000.0 and 0 < roof["frac"] < 1
     assert roof["algorithmic_bytes_per_launch"] == 2 * 8 * 30000 ** 2 + 2 * 8 * 29996 ** 2
-    assert set(d["hbm_measured"]) == {"copy", "triad", "sweep_mix"}
+    assert set(d["hbm_measured"]) == {"copy", "triad", "sweep_mix", "copy_best", "copy_best_shape"}
     rep = d["repeats"]
     assert rep["runs"] == 5 and rep["sweeps_each"] == 3 and len(rep["gbs"]) == 5
     assert sorted(rep["gbs"])[2] == rep["gbs_median"] and rep["gbs_median"] > 0
+    # the traffic figure belongs to the tiling that ran, or is null
+    assert roof["traffic_tiling"] and (roof["traffic"] is None or
+                                       roof["traffic_tiling"] in roof["traffic_profiled_tilings"])
+    f32 = d["fp32_storage"]
+    assert f32["validated"] is True and f32["points_per_s_vs_f64"] > 1.0
+    err = f32["error_vs_f64_seeded"]
+    assert 0 < err["normwise_l2"] < 1e-5 and err["target_normwise"] == 1e-6
+    bat = d["batched_config4"]
+    assert bat["solves_per_s"] > bat["one_by_one_f64_solves_per_s"] > 0
+    assert bat["max_rel_norm_diff_vs_f64"] < 1e-5
 
 
 @pytest.mark.gpu

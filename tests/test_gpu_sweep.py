@@ -310,3 +310,66 @@ def test_unfused_forms_equal_fused_sweep(gpu, dtype, pattern, radius):
         torch.cuda.synchronize()
         assert np.array_equal(bits(to_host(os_)), bits(to_host(out)))
         assert np.array_equal(bits(to_host(xs)), bits(to_host(b1)))
+
+
+# The register-window sweep (variants 6 and 7: the FP32/BF16 forms at
+# R <= 3, each row widened once into a ring of windows held in registers,
+# neighbour columns by DPP lane shifts): bitwise against the oracle for
+# star and grid R=1..3, one strip (lanes past n, both halo lanes at the
+# grid's edge) and several strips (interior strips whose neighbours come
+# from the halo vectors of the adjacent strips), row blocks from one row to
+# the whole grid, both pipeline depths.
+RW = [(p, r) for p in (0, 1) for r in (1, 2, 3)]
+
+
+@pytest.mark.parametrize("dtype", [F32, BF16])
+@pytest.mark.parametrize("pattern,radius", RW)
+def test_register_window_tilings(gpu, oracle, dtype, pattern, radius):
+    L = K.lib()
+    iters, seed = 3, 29
+    try:
+        for n in (40, 1104):  # multiples of the vector width: the whole-vector path
+            ref, ref_in = oracle.run_lp(dtype, pattern, radius, n, iters, 1, seed)
+            for v, rows in [(6, 1), (7, 2), (6, 7), (7, 16), (6, 64), (7, n)]:
+                print(f"regwin dtype={dtype} {NAME[pattern]}{radius} n={n} variant={v} rows={rows}", flush=True)
+                K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
+                b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+                K.init(K.INIT_SEEDED, seed, dtype, b0, out)
+                fb, _ = K.run(pattern, radius, dtype, iters, b0, b1, out)
+                assert np.array_equal(to_host(out).view(ref.dtype), ref), (n, v, rows)
+                assert np.array_equal(to_host((b0, b1)[fb]).view(ref_in.dtype), ref_in), (n, v, rows)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
+
+
+@pytest.mark.parametrize("dtype", [K.F64, F32, BF16])
+def test_frame_columns_keep_their_bits(gpu, oracle, dtype):
+    # out's frame (the first and last R rows and columns) is never written:
+    # a sentinel there survives every sweep variant, and the interior still
+    # matches the oracle (the frame of out feeds no interior point)
+    L = K.lib()
+    n, iters, seed = 616, 2, 8  # a multiple of every vector width: the whole-vector path
+    cases = [(0, 2, v) for v in (1, 4, 5)] + [(1, 3, 5), (0, 7, 4)]
+    if dtype != K.F64:
+        cases += [(1, 2, 6), (0, 3, 7)]
+    try:
+        for pattern, radius, v in cases:
+            print(f"frame dtype={dtype} {NAME[pattern]}{radius} variant={v}", flush=True)
+            K.check(L.prk_set_sweep_variant(v))
+            b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+            K.init(K.INIT_SEEDED, seed, dtype, b0, out)
+            frame = torch.ones((n, n), dtype=torch.bool, device=gpu)
+            frame[radius:n - radius, radius:n - radius] = False
+            out.view(torch.int16 if dtype == BF16 else torch.int32 if dtype == F32 else torch.int64)[frame] = -7
+            before = to_host(out).copy()
+            K.run(pattern, radius, dtype, iters, b0, b1, out)
+            got = to_host(out)
+            fr = frame.cpu().numpy()
+            assert np.array_equal(bits(got)[fr], bits(before)[fr]), (pattern, radius, v)
+            if dtype == K.F64:
+                ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, seed)
+            else:
+                ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, seed)
+            assert np.array_equal(bits(got)[~fr], bits(ref.view(got.dtype))[~fr]), (pattern, radius, v)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1))
