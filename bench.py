@@ -227,8 +227,15 @@ def bench_batch(a) -> int:
     specs = config4_solves()
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    if a.batch_heights:
+        # replay the tile heights another run recorded (its JSON line)
+        from kernels_amd.batch import tuning_set
+        rec = json.load(open(a.batch_heights))["tile_heights"]["by_pattern_radius_log2n"]
+        for key, rows in rec.items():
+            pr, k = key.split("/2^")
+            tuning_set(0 if pr.startswith("star") else 1, int(pr[4:]), 1, 1 << int(k), int(rows))
     t0 = time.perf_counter()
-    keys = 0 if a.no_tune else autotune(specs, sample=16, reps=2)
+    keys = 0 if (a.no_tune or a.batch_heights) else autotune(specs, sample=16, reps=2)
     t_tune = time.perf_counter() - t0
     heights = {}
     for p_, r_, d_, n_, *_ in specs:
@@ -844,6 +851,8 @@ def main(argv=None) -> int:
                     help="slab workloads: skip the whole-grid comparison sweeps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the sweeps, run one 2 GiB copy (known bytes) for counter calibration")
+    ap.add_argument("--batch-heights", default=None,
+                    help="batch4096_f32: replay the tile heights recorded in this bench JSON line (no tuning)")
     ap.add_argument("--no-extra", action="store_true",
                     help="default workload: skip the FP32-storage and batched (4096 grids) sections")
     a = ap.parse_args(argv)

@@ -273,6 +273,9 @@ void prk_batch_destroy(prk_batch *b);
  * prk_batch_tuning_get reports the height a grid would get. */
 int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps, int *keys_tuned);
 int prk_batch_tuning_get(int pattern, int radius, int dtype, int n, int *rows);
+/* Sets the tile height of the key of (pattern, radius, dtype, n), as
+ * prk_batch_autotune would (a recorded table can be replayed). */
+int prk_batch_tuning_set(int pattern, int radius, int dtype, int n, int rows);
 int prk_batch_tuning_clear(void);
 
 /* Tiles per round the schedules of later prk_batch_create calls aim for

@@ -87,6 +87,7 @@ SIGNATURES.update({
     "prk_set_batch_round_tiles": (c_int, [c_ll]),
     "prk_batch_autotune": (c_int, [_SOLVE_P, c_int, c_int, c_int, c_int_p]),
     "prk_batch_tuning_get": (c_int, [c_int, c_int, c_int, c_int, c_int_p]),
+    "prk_batch_tuning_set": (c_int, [c_int, c_int, c_int, c_int, c_int]),
     "prk_batch_tuning_clear": (c_int, []),
 })
 

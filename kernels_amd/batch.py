@@ -90,6 +90,10 @@ def tuning_get(pattern: int, radius: int, dtype: int, n: int) -> int:
     return r.value
 
 
+def tuning_set(pattern: int, radius: int, dtype: int, n: int, rows: int) -> None:
+    check(lib().prk_batch_tuning_set(pattern, radius, dtype, n, rows))
+
+
 def tuning_clear() -> None:
     check(lib().prk_batch_tuning_clear())
 

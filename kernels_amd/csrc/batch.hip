@@ -505,6 +505,23 @@ int prk_batch_tuning_get(int pattern, int radius, int dtype, int n, int *rows)
     return PRK_OK;
 }
 
+int prk_batch_tuning_set(int pattern, int radius, int dtype, int n, int rows)
+{
+    prk_solve s{};
+    s.pattern = pattern;
+    s.radius = radius;
+    s.dtype = dtype;
+    s.n = n;
+    s.iterations = 0;
+    s.init = PRK_INIT_ANALYTIC;
+    int rc = validate_solve(s);
+    if (rc) return rc;
+    if (rows < 1) return set_error(PRK_EINVAL, "tile height must be >= 1, got %d", rows);
+    std::lock_guard<std::mutex> lock(g_brows_mu);
+    g_brows[batch_key(s)] = rows;
+    return PRK_OK;
+}
+
 int prk_batch_tuning_clear(void)
 {
     std::lock_guard<std::mutex> lock(g_brows_mu);

@@ -31,7 +31,9 @@ struct DDv {
     double hi, lo;
 };
 
-struct BGrid {
+// 32-byte aligned (96 bytes): every workgroup reads its grid's entry with
+// scalar loads at the start of a tile.
+struct alignas(32) BGrid {
     void *buf0, *buf1, *out;
     long long pitch;         // elements, 16-byte multiple; padding may be written
     int n, pattern, radius;
