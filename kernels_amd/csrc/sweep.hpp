@@ -66,6 +66,7 @@ struct SweepArgs {
                         // column groups padded to a multiple of 8 (orders 3..7)
     int pad_writable;   // columns n..pitch-1 may be overwritten
     int order;          // tile order, see tile_of()
+    int stash;          // dynamic LDS holds the edge stash (kStashBytes)
 };
 
 // Workgroup -> tile.  Workgroup b runs on XCD b%8 (round-robin dispatch).
@@ -601,22 +602,125 @@ enum { MODE_ELEM = 0, MODE_VEC = 1, MODE_VEC_LD = 2 };
 // element LDS ring.  LDAUX adds cache policy bits to every HBM load (the
 // batch kernel bypasses L1 with it, since neighbouring tiles written by
 // other CUs are read there).
+// Store offsets of this lane's V elements of out (frame columns and columns
+// past n out of range) and whether all V are interior.  Computed where a
+// store needs them, from lane_now(): hoisted out of the row loop they raised
+// the register count of the sweep kernels by 2-8 VGPRs and put some
+// large-R ones past an occupancy step.
+// The wave's lane, counted by the hardware where it is needed: a volatile
+// asm is neither hoisted out of the row loop nor merged with an earlier
+// count, so nothing derived from it stays in a register across the loop.
+__device__ inline int lane_now()
+{
+    int lane;
+    __asm__ volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    return lane;
+}
+
+template <int V, int ES>
+__device__ inline bool out_offsets(int c0, int n, int R, unsigned (&off)[V])
+{
+    const int cm = c0 + lane_now() * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) off[v] = (cm + v >= R) && (cm + v < n - R) ? (unsigned)((cm + v) * ES) : kOOB;
+    return cm >= R && cm + V <= n - R;
+}
+
+// Deferred edge stores (FRAME_KEEP).  A lane whose vector holds both frame
+// and interior columns (the vector holding column R, and the one holding
+// column n-R-1 and n-R) can only store its interior elements one by one,
+// into a line whose frame bytes stay unwritten.  Such a partial store is
+// acknowledged late, and loads and stores share one counter (vmcnt), so a
+// wave that made one every row waited at its next rows' loads: the edge
+// workgroups ran up to twice as long as the others and the sweep lost up to
+// 15 % (FP32 grid R=6, BF16 star R=6 at n=16384).  Instead the edge lane parks its
+// result vector for output row j of the tile in an LDS stash (slots 2j and
+// 2j+1 for the left and right sides: one tile has at most one lane per side
+// that straddles a frame boundary), and lane j writes the
+// elements of row j, all rows at once, after the tile's last output row.
+constexpr int kStashRows = 64;
+constexpr size_t kStashBytes = 2 * kStashRows * sizeof(u32x4);
+
+// In the row loop, edge waves: lanes whose vector is interior store it whole;
+// the others either park it in the stash (`stash` non-null) or store their
+// interior elements now.
+template <typename T, int V, int ES, int AUX>
+__device__ inline void edge_store(__amdgpu_buffer_rsrc_t rout, unsigned off_main, const u32x4 &resv, int c0,
+                                  int n, int R, u32x4 *stash, int row)
+{
+    if (stash) {
+        // [cm, cm+V) against the interior [R, n-R); a straddling vector
+        // starts in the frame on the left side (slot 2 row), in the interior
+        // on the right (slot 2 row + 1)
+        const int cm = c0 + lane_now() * V;
+        const bool whole = cm >= R && cm + V <= n - R;
+        bstore16<AUX>(rout, whole ? off_main : kOOB, resv);
+        if (!whole && cm < n - R && cm + V > R) stash[2 * row + (cm < R ? 0 : 1)] = resv;
+    } else {
+        unsigned off[V];
+        const bool whole = out_offsets<V, ES>(c0, n, R, off);
+        bstore16<AUX>(rout, whole ? off_main : kOOB, resv);
+        union { u32x4 v; T e[V]; } r;
+        r.v = resv;
+#pragma unroll
+        for (int v = 0; v < V; ++v) bstore_elem<T, AUX>(rout, whole ? kOOB : off[v], r.e[v]);
+    }
+}
+
+// After the row loop: lane j writes the parked interior elements of output
+// row olo + j (nout <= kStashRows rows; the caller keeps kStashRows rows of
+// out under 2 GiB, so every offset fits one resource).
+template <typename T, int V, int ES, int AUX>
+__device__ inline void flush_edge_stash(const u32x4 *stash, T *out, long long pitch, int n, int R, int c0, int sw,
+                                        int olo, int nout)
+{
+    const int lane = lane_now();
+    // the stash writes of this wave land before its reads (in order per
+    // wave); the fences keep the compiler from moving them
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const __amdgpu_buffer_rsrc_t r =
+        row_rsrc(out + (long long)olo * pitch, (unsigned)(((long long)(nout - 1) * pitch + n) * ES));
+    const int cb_left = R - R % V;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int col = side == 0 ? R : n - R;  // first interior / first right frame column
+        const int cb = col - col % V;           // the vector holding it
+        if (col % V == 0 || cb < c0 || cb >= c0 + sw) continue;
+        if (side == 1 && cb == cb_left) continue;  // one vector holds both boundaries
+        if (lane < nout) {
+            union { u32x4 v; T e[V]; } x;
+            x.v = stash[2 * lane + side];
+            const unsigned row_off = (unsigned)((long long)lane * pitch * ES);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int c = cb + v;
+                bstore_elem<T, AUX>(r, c >= R && c < n - R ? row_off + (unsigned)(c * ES) : kOOB, x.e[v]);
+            }
+        }
+    }
+}
+
 // How a wave stores out (whole-vector mode).  In the first and last strips
-// of a row a lane's vector may mix interior and frame columns:
+// of a row (`edge`, wave-uniform) a lane's vector may mix interior and frame
+// columns:
 //   FRAME_KEEP     such lanes store their interior elements one by one
 //                  (frame elements out of range): the frame is never
-//                  written, as the reference's operators promise;
+//                  written, as the reference's operators promise.  Waves of
+//                  the other strips store whole vectors; the test is one
+//                  scalar branch at the store.
 //   FRAME_REWRITE  every lane stores the whole vector with the frame
 //                  elements as they were loaded (grids the library owns,
 //                  where nothing else writes the frame: one store instead
-//                  of V, and no edge copy of the tile code).
-// FRAME_INTERIOR waves (every other strip) hold interior columns only.
-enum { FRAME_INTERIOR = 0, FRAME_KEEP = 1, FRAME_REWRITE = 2 };
+//                  of V).
+enum { FRAME_KEEP = 1, FRAME_REWRITE = 2 };
 
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX, int FRAME>
 __device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
                                       const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                      typename DTypeTraits<DT>::T *ring_base, int lane)
+                                      typename DTypeTraits<DT>::T *ring_base, int lane, bool edge,
+                                      u32x4 *stash_lds)
 {
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
@@ -629,6 +733,8 @@ __device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out
 
     const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
     const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    // edge waves park their straddling vectors (tiles of at most kStashRows output rows)
+    u32x4 *const stash = FRAME == FRAME_KEEP && MODE == MODE_VEC && ohi - olo <= kStashRows ? stash_lds : nullptr;
     const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
 
     const T *in = (const T *)in_;
@@ -646,17 +752,6 @@ __device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out
     const int lh = lane < NH ? lane * V : HP + SW + (lane - NH) * V;  // LDS column
     const unsigned off_main = (unsigned)(cm * ES);
     const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;  // ch<0 wraps out of range
-    // Per-column store offsets of out (EDGE waves, and element modes): frame
-    // columns go out of range and are never written.
-    unsigned off_out[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
-    // EDGE waves: lanes whose whole vector is interior still store it whole;
-    // the others store their interior elements one by one
-    const bool vec_interior = cm >= R && cm + V <= n - R;
-    unsigned off_part[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
 
     T(*ring)[LR] = (T(*)[LR])ring_base;
 
@@ -737,19 +832,25 @@ __device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out
                 union { u32x4 v; T e[V]; } res;
                 compute_points<DT, P, R, V, HP>(rows, cur.outv.e, res.e);
                 const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
-                if constexpr (MODE == MODE_VEC && FRAME == FRAME_INTERIOR) {
-                    bstore16<AUX_STREAM>(rout, off_main, res.v);
-                } else if constexpr (MODE == MODE_VEC && FRAME == FRAME_REWRITE) {
+                if constexpr (MODE == MODE_VEC && FRAME == FRAME_REWRITE) {
+                    unsigned off[V];
+                    out_offsets<V, ES>(c0, n, R, off);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) res.e[v] = off_out[v] != kOOB ? res.e[v] : cur.outv.e[v];
+                    for (int v = 0; v < V; ++v) res.e[v] = off[v] != kOOB ? res.e[v] : cur.outv.e[v];
                     bstore16<AUX_STREAM>(rout, off_main, res.v);
                 } else if constexpr (MODE == MODE_VEC) {
-                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
-#pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
+                    if (!edge) {
+                        bstore16<AUX_STREAM>(rout, off_main, res.v);
+                    } else {
+                        edge_store<T, V, ES, AUX_STREAM>(rout, off_main, res.v, c0, n, R, stash, o - olo);
+                        if (stash && o + 1 == ohi)
+                            flush_edge_stash<T, V, ES, AUX_STREAM>(stash, out, pitch, n, R, c0, SW, olo, ohi - olo);
+                    }
                 } else {
+                    unsigned off[V];
+                    out_offsets<V, ES>(c0, n, R, off);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off[v], res.e[v]);
                 }
             }
             slot = (slot + 1 == S) ? 0 : slot + 1;
@@ -759,28 +860,16 @@ __device__ inline void wave_tile_impl(const void *in_, void *in_next_, void *out
 
 // KEEP_FRAME false: the grid belongs to the library (batched and served
 // solves), so every wave stores whole vectors, the frame elements of out as
-// they were loaded (FRAME_REWRITE), from one copy of the tile code: the
-// batch runs dozens of these kernels side by side, and with an edge and an
-// interior copy of every tile loop it ran 8 % slower (same box, alternating
-// runs).
+// they were loaded (FRAME_REWRITE).
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, int LDAUX, bool KEEP_FRAME = true>
 __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, long long pitch, int n,
                                  const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                 typename DTypeTraits<DT>::T *ring_base, int lane)
+                                 typename DTypeTraits<DT>::T *ring_base, int lane, u32x4 *stash_lds = nullptr)
 {
-    if constexpr (!KEEP_FRAME) {
-        wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_REWRITE>(in_, in_next_, out_, pitch, n, sp, c0,
-                                                                           r0, r1, pad_writable, ring_base, lane);
-    } else {
-        // wave-uniform: only the first and last strips of a row hold frame columns
-        if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
-            wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_KEEP>(in_, in_next_, out_, pitch, n, sp, c0,
-                                                                            r0, r1, pad_writable, ring_base, lane);
-        else
-            wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, FRAME_INTERIOR>(in_, in_next_, out_, pitch, n, sp,
-                                                                                c0, r0, r1, pad_writable,
-                                                                                ring_base, lane);
-    }
+    // wave-uniform: only the first and last strips of a row hold frame columns
+    const bool edge = c0 < R || c0 + SweepGeom<DT, R>::SW > n - R;
+    wave_tile_impl<DT, P, R, FUSE, MODE, PF, NT, LDAUX, KEEP_FRAME ? FRAME_KEEP : FRAME_REWRITE>(
+        in_, in_next_, out_, pitch, n, sp, c0, r0, r1, pad_writable, ring_base, lane, edge, stash_lds);
 }
 
 // wave_tile for the star radii of star_reg_ring: the same loads, stores and
@@ -788,11 +877,13 @@ __device__ inline void wave_tile(const void *in_, void *in_next_, void *out_, lo
 // slots (M >= 2R+1+PF, a multiple of PF; row ri in slot (ri - lo) % M; the
 // row loop is unrolled by M so every slot index is a constant) and an LDS
 // ring of R+1 rows for the centre rows.
-template <int DT, int R, bool FUSE, int MODE, int PF, int NT, bool EDGE>
-__device__ inline void wave_tile_star_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                           const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                           typename DTypeTraits<DT>::T *ring_base, int lane)
+template <int DT, int R, bool FUSE, int MODE, int PF, int NT>
+__device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
+                                      typename DTypeTraits<DT>::T *ring_base, int lane, u32x4 *stash_lds)
 {
+    // wave-uniform: only the first and last strips of a row hold frame columns
+    const bool edge = c0 < R || c0 + SweepGeom<DT, R>::SW > n - R;
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
     constexpr int V = G::V, HP = G::HP, NH = G::NH, LR = G::LR, S = G::S;
@@ -803,6 +894,8 @@ __device__ inline void wave_tile_star_impl(const void *in_, void *in_next_, void
 
     const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
     const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    // edge waves park their straddling vectors (tiles of at most kStashRows output rows)
+    u32x4 *const stash = MODE == MODE_VEC && ohi - olo <= kStashRows ? stash_lds : nullptr;
     const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
 
     const T *in = (const T *)in_;
@@ -816,15 +909,6 @@ __device__ inline void wave_tile_star_impl(const void *in_, void *in_next_, void
     const int lh = lane < NH ? lane * V : HP + G::SW + (lane - NH) * V;
     const unsigned off_main = (unsigned)(cm * ES);
     const unsigned off_halo = lane < 2 * NH ? (unsigned)(ch * ES) : kOOB;
-    unsigned off_out[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
-    // EDGE waves: lanes whose whole vector is interior still store it whole;
-    // the others store their interior elements one by one
-    const bool vec_interior = cm >= R && cm + V <= n - R;
-    unsigned off_part[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
 
     T(*ring)[LR] = (T(*)[LR])ring_base;
 
@@ -905,33 +989,24 @@ __device__ inline void wave_tile_star_impl(const void *in_, void *in_next_, void
                 Vec res;
                 star_points<DT, R, V, HP>(&ring[cs][HP + lane * V], col, sd.outv.e, res.e);
                 const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
-                if constexpr (MODE == MODE_VEC && !EDGE) {
-                    bstore16<AUX_STREAM>(rout, off_main, res.v);
-                } else if constexpr (MODE == MODE_VEC) {
-                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
-#pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
+                if constexpr (MODE == MODE_VEC) {
+                    if (!edge) {
+                        bstore16<AUX_STREAM>(rout, off_main, res.v);
+                    } else {
+                        edge_store<T, V, ES, AUX_STREAM>(rout, off_main, res.v, c0, n, R, stash, o - olo);
+                        if (stash && o + 1 == ohi)
+                            flush_edge_stash<T, V, ES, AUX_STREAM>(stash, out, pitch, n, R, c0, G::SW, olo, ohi - olo);
+                    }
                 } else {
+                    unsigned off[V];
+                    out_offsets<V, ES>(c0, n, R, off);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off[v], res.e[v]);
                 }
             }
             lslot = lslot + 1 == L ? 0 : lslot + 1;
         }
     }
-}
-
-template <int DT, int R, bool FUSE, int MODE, int PF, int NT>
-__device__ inline void wave_tile_star(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                      const RowSpan &sp, int c0, int r0, int r1, int pad_writable,
-                                      typename DTypeTraits<DT>::T *ring_base, int lane)
-{
-    if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
-        wave_tile_star_impl<DT, R, FUSE, MODE, PF, NT, true>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
-                                                             pad_writable, ring_base, lane);
-    else
-        wave_tile_star_impl<DT, R, FUSE, MODE, PF, NT, false>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
-                                                              pad_writable, ring_base, lane);
 }
 
 
@@ -1038,10 +1113,13 @@ __device__ inline void rw_pair_sum(const Slot (&W)[S], int k, float (&acc)[V])
     if constexpr (A < R) rw_pair_sum<P, R, V, S, A + 1>(W, k, acc);
 }
 
-template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT, bool EDGE>
-__device__ inline void wave_tile_rw_impl(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                         const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane)
+template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
+__device__ inline void wave_tile_rw(const void *in_, void *in_next_, void *out_, long long pitch, int n,
+                                    const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane,
+                                    u32x4 *stash_lds)
 {
+    // wave-uniform: only the first and last strips of a row hold frame columns
+    const bool edge = c0 < R || c0 + SweepGeom<DT, R>::SW > n - R;
     typedef SweepGeom<DT, R> G;
     typedef typename G::T T;
     constexpr int V = G::V, HP = G::HP, NH = G::NH, S = G::S;
@@ -1052,6 +1130,8 @@ __device__ inline void wave_tile_rw_impl(const void *in_, void *in_next_, void *
 
     const int lo = max(r0 - R, 0), hi = min(r1 + R, sp.nrows);
     const int olo = max(r0, R), ohi = min(r1, sp.nrows - R);
+    // edge waves park their straddling vectors (tiles of at most kStashRows output rows)
+    u32x4 *const stash = MODE == MODE_VEC && ohi - olo <= kStashRows ? stash_lds : nullptr;
     const int ulo = max(r0, sp.upd_lo), uhi = min(r1, sp.upd_hi);
 
     const T *in = (const T *)in_;
@@ -1066,15 +1146,6 @@ __device__ inline void wave_tile_rw_impl(const void *in_, void *in_next_, void *
     const int ch = lane == 0 ? c0 - V : c0 + G::SW;
     const unsigned off_main = (unsigned)(cm * ES);
     const unsigned off_halo = lane < 2 ? (unsigned)(ch * ES) : kOOB;  // ch < 0 wraps out of range
-    unsigned off_out[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_out[v] = (cm + v >= R) && (cm + v < n - R) ? off_main + v * ES : kOOB;
-    // EDGE waves: lanes whose whole vector is interior still store it whole;
-    // the others store their interior elements one by one
-    const bool vec_interior = cm >= R && cm + V <= n - R;
-    unsigned off_part[V];
-#pragma unroll
-    for (int v = 0; v < V; ++v) off_part[v] = vec_interior ? kOOB : off_out[v];
 
     // Row ri's loads (main, halo, out row ri - R) in raw[(ri - lo) % S]; its
     // window in W[(ri - lo) % S].
@@ -1143,32 +1214,29 @@ __device__ inline void wave_tile_rw_impl(const void *in_, void *in_next_, void *
                 union { u32x4 v; T e[V]; } res;
                 from_f32_vec<V>(f, res.e);
                 const __amdgpu_buffer_rsrc_t rout = row_rsrc(out + (long long)o * pitch, st_bytes);
-                if constexpr (MODE == MODE_VEC && !EDGE) {
-                    bstore16<AUX_STREAM>(rout, off_main, res.v);
-                } else if constexpr (MODE == MODE_VEC) {
-                    bstore16<AUX_STREAM>(rout, vec_interior ? off_main : kOOB, res.v);
-#pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_part[v], res.e[v]);
+                if constexpr (MODE == MODE_VEC) {
+                    if (!edge) {
+                        bstore16<AUX_STREAM>(rout, off_main, res.v);
+                    } else {
+                        edge_store<T, V, ES, AUX_STREAM>(rout, off_main, res.v, c0, n, R, stash, o - olo);
+                        if (stash && o + 1 == ohi)
+                            flush_edge_stash<T, V, ES, AUX_STREAM>(stash, out, pitch, n, R, c0, G::SW, olo, ohi - olo);
+                    }
                 } else {
+                    unsigned off[V];
+                    out_offsets<V, ES>(c0, n, R, off);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off_out[v], res.e[v]);
+                    for (int v = 0; v < V; ++v) bstore_elem<T, AUX_STREAM>(rout, off[v], res.e[v]);
                 }
             }
         }
     }
 }
 
-template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
-__device__ inline void wave_tile_rw(const void *in_, void *in_next_, void *out_, long long pitch, int n,
-                                    const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane)
-{
-    if (MODE == MODE_VEC && (c0 < R || c0 + SweepGeom<DT, R>::SW > n - R))
-        wave_tile_rw_impl<DT, P, R, FUSE, MODE, PF, NT, true>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
-                                                              pad_writable, lane);
-    else
-        wave_tile_rw_impl<DT, P, R, FUSE, MODE, PF, NT, false>(in_, in_next_, out_, pitch, n, sp, c0, r0, r1,
-                                                               pad_writable, lane);
-}
+
+// Dynamic LDS of the sweep kernels: the launch's occupancy padding, whose
+// first kStashBytes hold the edge stash when SweepArgs::stash is set.
+extern __shared__ __attribute__((aligned(16))) u32x4 sweep_dyn_lds[];
 
 // The register-window sweep kernel: the launch geometry of sweep_kernel, no LDS.
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
@@ -1182,8 +1250,10 @@ __global__ __launch_bounds__(kThreads) void sweep_rw_kernel(SweepArgs a)
     if (strip >= a.nstrips) return;
     const int r0 = rb * a.rows_per_wave;
     const int r1 = min(r0 + a.rows_per_wave, a.span.nrows);
+    // no edge stash: the register-window sweep's edge waves showed no cost
+    // from storing their straddling vectors at once
     wave_tile_rw<DT, P, R, FUSE, MODE, PF, NT>(a.in, a.in_next, a.out, a.pitch, a.n, a.span,
-                                               strip * SweepGeom<DT, R>::SW, r0, r1, a.pad_writable, lane);
+                                               strip * SweepGeom<DT, R>::SW, r0, r1, a.pad_writable, lane, nullptr);
 }
 
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
@@ -1207,10 +1277,12 @@ void sweep_kernel(SweepArgs a)
     const int r1 = min(r0 + a.rows_per_wave, a.span.nrows);
     if constexpr (star_reg_ring<DT, P, R>())
         wave_tile_star<DT, R, FUSE, MODE, PF, NT>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW, r0,
-                                                  r1, a.pad_writable, &lds[wave][0][0], lane);
+                                                  r1, a.pad_writable, &lds[wave][0][0], lane,
+                                                  a.stash ? sweep_dyn_lds : nullptr);
     else
         wave_tile<DT, P, R, FUSE, MODE, PF, NT, 0>(a.in, a.in_next, a.out, a.pitch, a.n, a.span, strip * G::SW,
-                                                   r0, r1, a.pad_writable, &lds[wave][0][0], lane);
+                                                   r0, r1, a.pad_writable, &lds[wave][0][0], lane,
+                                                   a.stash ? sweep_dyn_lds : nullptr);
 }
 
 }  // namespace prk

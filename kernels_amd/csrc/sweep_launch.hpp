@@ -1,6 +1,8 @@
 // Instantiation and launch of the sweep kernel for one storage dtype.
 #pragma once
 
+#include <algorithm>
+
 #include "prk_internal.hpp"
 #include "sweep.hpp"
 
@@ -39,9 +41,31 @@ int launch_one(const SweepPlan &p, const void *in, void *in_next, void *out, hip
         blocks = 8u * (unsigned)a.chunk;
     }
     // Extra dynamic LDS caps the workgroups resident per CU (occupancy knob).
-    const size_t lds_cap = p.wgs_per_cu > 0 ? (160u * 1024u) / (unsigned)p.wgs_per_cu : 0;
+    constexpr size_t kLdsPerCu = 160u * 1024u;
+    const size_t lds_cap = p.wgs_per_cu > 0 ? kLdsPerCu / (unsigned)p.wgs_per_cu : 0;
     constexpr size_t lds_bytes = RW ? 0 : sweep_lds_bytes<DT, P, R>();
-    const size_t dyn = lds_cap > lds_bytes ? lds_cap - lds_bytes : 0;
+    size_t dyn = lds_cap > lds_bytes ? lds_cap - lds_bytes : 0;
+    // The edge stash of the LDS-ring whole-vector sweeps lives at the start
+    // of the dynamic LDS: in the occupancy padding when it is large enough,
+    // else in kStashBytes added to it where that leaves the workgroups per CU
+    // as they were (at most 8: 32 waves per CU), else not at all (edge waves
+    // store their straddling vectors at once).  Its flush addresses
+    // kStashRows rows of out from one resource, which must stay below the
+    // out-of-range offset.
+    a.stash = 0;
+    if (!RW && MODE == MODE_VEC &&
+        (long long)kStashRows * p.pitch * (long long)sizeof(typename G::T) < (long long)kOOB) {
+        if (dyn >= kStashBytes) {
+            a.stash = 1;
+        } else {
+            const size_t wgs_before = std::min<size_t>(8, kLdsPerCu / std::max<size_t>(lds_bytes + dyn, 1));
+            const size_t wgs_after = kLdsPerCu / (lds_bytes + kStashBytes);
+            if (wgs_after >= wgs_before) {
+                dyn = kStashBytes;
+                a.stash = 1;
+            }
+        }
+    }
     if constexpr (RW)
         hipLaunchKernelGGL((sweep_rw_kernel<DT, P, R, FUSE, MODE, PF, NT>), dim3(blocks), dim3(kThreads), dyn, s, a);
     else

@@ -373,3 +373,52 @@ def test_frame_columns_keep_their_bits(gpu, oracle, dtype):
             assert np.array_equal(bits(got)[~fr], bits(ref.view(got.dtype))[~fr]), (pattern, radius, v)
     finally:
         K.check(L.prk_set_sweep_variant(-1))
+
+
+# Edge waves park the vectors that straddle a frame boundary (the one holding
+# column R and the one holding columns n-R-1, n-R) in LDS and write their
+# interior elements after the row loop when the tile has at most 64 output
+# rows and the launch has LDS room for the stash; taller tiles and the
+# register-window sweep store them at once.  Cases: one boundary per edge
+# wave, both boundaries in one wave (n <= 64 V), a second edge strip that
+# holds frame columns only (n = 130 in FP64), register-ring and LDS-ring
+# stars, grids; tilings of 63, 64 and 65 rows, occupancy caps that leave the
+# stash in the padding or add it, padded pitch.  The frame keeps a sentinel
+# and the interior matches the oracle bit for bit.
+STASH = [(K.F64, 0, 3, 616), (K.F64, 1, 5, 130), (K.F64, 0, 7, 90), (F32, 1, 3, 616), (F32, 0, 6, 40),
+         (BF16, 0, 5, 616), (BF16, 1, 3, 104)]
+
+
+@pytest.mark.parametrize("dtype,pattern,radius,n", STASH)
+def test_edge_stash_paths(gpu, oracle, dtype, pattern, radius, n):
+    L = K.lib()
+    iters, seed = 2, 13
+    V = 16 // np.dtype(np.float64 if dtype == K.F64 else np.float32 if dtype == F32 else np.uint16).itemsize
+    if dtype == K.F64:
+        ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, seed)
+    else:
+        ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, seed)
+    tilings = [(4, 64, 2, None), (5, 64, 0, None), (4, 63, 4, None), (4, 65, 3, None), (1, 100, 2, None),
+               (5, 8, 3, n + 4 * V), (2, 1, 0, None)]
+    if dtype != K.F64 and radius <= 3:
+        tilings.append((7, 64, 2, None))
+    try:
+        for v, rows, wgs, pitch in tilings:
+            print(f"stash dtype={dtype} {NAME[pattern]}{radius} n={n} variant={v} rows={rows} wgs={wgs} "
+                  f"pitch={pitch}", flush=True)
+            K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
+            K.check(L.prk_set_sweep_order(1, wgs))
+            b0, b1, out = (K.alloc_grid(n, dtype, pitch=pitch) for _ in range(3))
+            K.init(K.INIT_SEEDED, seed, dtype, b0, out)
+            frame = torch.ones((n, n), dtype=torch.bool, device=gpu)
+            frame[radius:n - radius, radius:n - radius] = False
+            out.view(torch.int16 if dtype == BF16 else torch.int32 if dtype == F32 else torch.int64)[frame] = -7
+            before = to_host(out).copy()
+            K.run(pattern, radius, dtype, iters, b0, b1, out)
+            got = to_host(out)
+            fr = frame.cpu().numpy()
+            assert np.array_equal(bits(got)[fr], bits(before)[fr]), (v, rows, wgs, pitch)
+            assert np.array_equal(bits(got)[~fr], bits(ref.view(got.dtype))[~fr]), (v, rows, wgs, pitch)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
+        K.check(L.prk_set_sweep_order(-1, -1))
