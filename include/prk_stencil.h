@@ -325,6 +325,7 @@ typedef struct prk_server_stats {
     long long pool_used, budget;
     double build_seconds; /* host time spent building and capturing windows */
     double wait_seconds;  /* host time spent waiting for windows to finish */
+    long long graph_updates; /* windows whose graph exec took new arguments in place instead of being rebuilt */
 } prk_server_stats;
 
 int prk_server_create(const prk_server_config *cfg, prk_server **out);

@@ -103,7 +103,8 @@ class ServerResult(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(f, c_ll) for f in ("submitted", "served", "rejected", "windows", "launches", "pending", "active",
-                                    "pool_used", "budget")] + [("build_seconds", c_double), ("wait_seconds", c_double)]
+                                    "pool_used", "budget")] + [("build_seconds", c_double), ("wait_seconds", c_double),
+                                                               ("graph_updates", c_ll)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

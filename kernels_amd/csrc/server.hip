@@ -120,8 +120,8 @@ struct prk_server {
     size_t d_tab_cap[2] = {0, 0};
     void *h_tab[2] = {nullptr, nullptr};
     size_t h_tab_cap[2] = {0, 0};
+    size_t norm_off[2] = {0, 0};  // finishing requests' norms in the slot's tables
     hipGraphExec_t exec[2] = {nullptr, nullptr};
-    hipGraph_t graph[2] = {nullptr, nullptr};
 
     std::mutex mu;
     std::condition_variable cv_work, cv_idle;
@@ -211,12 +211,12 @@ void retire(prk_server *s, int slot)
         fin.swap(s->inflight[slot]);
     }
     const double t = now_s();
-    for (Req *r : fin) {
-        double norm = 0;
-        if (hipMemcpy(&norm, r->g.norm, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
-            fail(s, set_error(PRK_EHIP, "server: norm copy failed"));
-        }
-        r->norm = norm;
+    // the window's one copy brought its finishing requests' norms into the
+    // slot's pinned staging, in the order of `fin`
+    const double *norms = (const double *)((const char *)s->h_tab[slot] + s->norm_off[slot]);
+    for (size_t i = 0; i < fin.size(); ++i) {
+        Req *r = fin[i];
+        r->norm = norms[i];
         r->t_done = t;
         r->done = true;
     }
@@ -286,6 +286,7 @@ int build_window(prk_server *s, long long w)
     };
     std::vector<Launch> launches[kBatchClasses];
     std::vector<Req *> finishing;
+    std::vector<size_t> finishing_grid;  // their entries in `grids`
     for (int round = 0; round < kRoundsPerWindow; ++round) {
         std::vector<RWork> per[kBatchClasses];
         bool any = false;
@@ -299,6 +300,7 @@ int build_window(prk_server *s, long long w)
             if (r->stage == last) {
                 r->final_window = (int)w;
                 finishing.push_back(r);
+                finishing_grid.push_back(i);
             }
             ++r->stage;
             any = true;
@@ -318,12 +320,18 @@ int build_window(prk_server *s, long long w)
         s->active.swap(keep);
         s->inflight[slot] = finishing;
     }
+    // tables: grids | work | the finishing requests' norms, which their
+    // final-norm stage writes there so that one copy brings them all back
     const size_t gbytes = align_up(sizeof(BGrid) * grids.size(), 256);
-    const size_t wbytes = sizeof(RWork) * work.size();
-    int rc = ensure_capacity(s, slot, gbytes + wbytes + 256);
+    const size_t wbytes = align_up(sizeof(RWork) * work.size(), 256);
+    const size_t nbytes = sizeof(double) * finishing.size();
+    int rc = ensure_capacity(s, slot, gbytes + wbytes + nbytes + 256);
     if (rc) return rc;
+    double *d_norms = (double *)((char *)s->d_tab[slot] + gbytes + wbytes);
+    for (size_t k = 0; k < finishing_grid.size(); ++k) grids[finishing_grid[k]].norm = d_norms + k;
+    s->norm_off[slot] = gbytes + wbytes;
     std::memcpy(s->h_tab[slot], grids.data(), sizeof(BGrid) * grids.size());
-    std::memcpy((char *)s->h_tab[slot] + gbytes, work.data(), wbytes);
+    std::memcpy((char *)s->h_tab[slot] + gbytes, work.data(), sizeof(RWork) * work.size());
     rc = check_hip(hipMemcpyAsync(s->d_tab[slot], s->h_tab[slot], gbytes + wbytes, hipMemcpyHostToDevice, s->stream),
                    "server table upload");
     if (rc) return rc;
@@ -331,10 +339,6 @@ int build_window(prk_server *s, long long w)
     const RWork *d_work = (const RWork *)((char *)s->d_tab[slot] + gbytes);
 
     // capture: fork one branch per class, its rounds in order, join
-    if (s->exec[slot]) (void)hipGraphExecDestroy(s->exec[slot]);
-    if (s->graph[slot]) (void)hipGraphDestroy(s->graph[slot]);
-    s->exec[slot] = nullptr;
-    s->graph[slot] = nullptr;
     rc = check_hip(hipStreamBeginCapture(s->capture, hipStreamCaptureModeThreadLocal), "server capture");
     if (rc) return rc;
     rc = check_hip(hipEventRecord(s->fork, s->capture), "server fork");
@@ -351,16 +355,35 @@ int build_window(prk_server *s, long long w)
     }
     hipGraph_t g = nullptr;
     const int rc2 = check_hip(hipStreamEndCapture(s->capture, &g), "server end capture");
-    s->graph[slot] = g;
     if (!rc) rc = rc2;
-    if (!rc) rc = check_hip(hipGraphInstantiate(&s->exec[slot], g, nullptr, nullptr, 0), "server instantiate");
+    // The slot's exec last ran window w-2, which has finished.  Where the new
+    // graph has its topology (the same classes, the same rounds each) the
+    // exec takes the new kernel arguments in place; otherwise it is replaced.
+    bool updated = false;
+    if (!rc && s->exec[slot]) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(s->exec[slot], g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();  // a refused update is not an error of the next call
+            (void)hipGraphExecDestroy(s->exec[slot]);
+            s->exec[slot] = nullptr;
+        }
+    }
+    if (!rc && !updated) rc = check_hip(hipGraphInstantiate(&s->exec[slot], g, nullptr, nullptr, 0), "server instantiate");
+    if (g) (void)hipGraphDestroy(g);
     if (!rc) rc = check_hip(hipGraphLaunch(s->exec[slot], s->stream), "server graph launch");
+    if (!rc && nbytes)
+        rc = check_hip(hipMemcpyAsync((char *)s->h_tab[slot] + s->norm_off[slot], d_norms, nbytes,
+                                      hipMemcpyDeviceToHost, s->stream),
+                       "server norm copy");
     if (!rc) rc = check_hip(hipEventRecord(s->window_done[slot], s->stream), "server window event");
     size_t nl = 0;
     for (auto &v : launches) nl += v.size();
     std::lock_guard<std::mutex> lock(s->mu);
     s->st.windows += 1;
     s->st.launches += (long long)nl;
+    if (updated) s->st.graph_updates += 1;
     s->st.build_seconds += now_s() - t_build;
     return rc;
 }
@@ -438,7 +461,6 @@ void server_free(prk_server *s)
     if (!s) return;
     for (int k = 0; k < 2; ++k) {
         if (s->exec[k]) (void)hipGraphExecDestroy(s->exec[k]);
-        if (s->graph[k]) (void)hipGraphDestroy(s->graph[k]);
         if (s->window_done[k]) (void)hipEventDestroy(s->window_done[k]);
         if (s->d_tab[k]) (void)hipFree(s->d_tab[k]);
         if (s->h_tab[k]) (void)hipHostFree(s->h_tab[k]);

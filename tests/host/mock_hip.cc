@@ -136,6 +136,7 @@ static void check_idle_locked(const void *p, size_t bytes, const char *op)
 }
 // Failure injection: the instantiation with this ordinal (1-based) fails.
 long long g_instantiations = 0, g_fail_instantiation = 0;
+long long g_updates = 0;  // exec updates in place
 
 #define MOCK_CHECK(c, ...)                                                            \
     do {                                                                              \
@@ -334,6 +335,38 @@ hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *
     }
     *e = (hipGraphExec_t) new std::vector<Launch>(*(std::vector<Launch> *)g);
     return hipSuccess;
+}
+// An exec takes a graph's arguments in place when the graph has its
+// launches' classes in the same order (the same topology).  While a failure
+// is armed for an instantiation every update is refused, so that each window
+// instantiates and the armed ordinal is reached.
+static thread_local hipError_t g_last_error = hipSuccess;
+hipError_t hipGraphExecUpdate(hipGraphExec_t e, hipGraph_t g, hipGraphNode_t *bad, hipGraphExecUpdateResult *res)
+{
+    {
+        std::lock_guard<std::mutex> l(g_mu);
+        check_idle_locked(e, 1, "hipGraphExecUpdate");
+    }
+    auto &ex = *(std::vector<Launch> *)e;
+    const auto &ng = *(const std::vector<Launch> *)g;
+    bool same = g_fail_instantiation == 0 && ex.size() == ng.size();
+    for (size_t i = 0; same && i < ng.size(); ++i) same = ex[i].cls == ng[i].cls;
+    if (bad) *bad = nullptr;
+    if (!same) {
+        if (res) *res = hipGraphExecUpdateErrorTopologyChanged;
+        g_last_error = hipErrorGraphExecUpdateFailure;
+        return hipErrorGraphExecUpdateFailure;
+    }
+    ex = ng;
+    ++prk::g_updates;
+    if (res) *res = hipGraphExecUpdateSuccess;
+    return hipSuccess;
+}
+hipError_t hipGetLastError(void)
+{
+    const hipError_t e = g_last_error;
+    g_last_error = hipSuccess;
+    return e;
 }
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t)
 {

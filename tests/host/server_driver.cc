@@ -25,7 +25,7 @@
 namespace prk {
 extern long long g_items, g_final;
 extern int g_check_all;
-extern long long g_instantiations, g_fail_instantiation;
+extern long long g_instantiations, g_fail_instantiation, g_updates;
 }  // namespace prk
 
 static prk_solve random_solve(std::mt19937_64 &rng, int nmin, int nmax, int rmax = 4)
@@ -93,8 +93,8 @@ int main(int argc, char **argv)
         fails += check_results(s, ids, "open loop");
         prk_server_stats st{};
         prk_server_stats_get(s, &st);
-        std::printf("open loop: refused %d, windows %lld, launches %lld, pool in use %lld\n", refused, st.windows,
-                    st.launches, st.pool_used);
+        std::printf("open loop: refused %d, windows %lld, launches %lld, graph updates %lld, pool in use %lld\n",
+                    refused, st.windows, st.launches, st.graph_updates, st.pool_used);
         fails += st.pool_used != 0 || st.served != (long long)ids.size();
         prk_server_destroy(s);
     }
@@ -207,6 +207,7 @@ int main(int argc, char **argv)
         prk_server_destroy(s);
         prk::g_fail_instantiation = 0;
     }
-    std::printf("entries %lld, final norms %lld, failures %d\n", prk::g_items, prk::g_final, fails);
+    std::printf("entries %lld, final norms %lld, graph updates %lld, failures %d\n", prk::g_items, prk::g_final,
+                prk::g_updates, fails);
     return fails ? 1 : 0;
 }

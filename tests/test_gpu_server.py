@@ -109,3 +109,22 @@ def test_arrivals_while_running(gpu, oracle):
         assert abs(res[rid]["norm"] - norm) <= 1e-13 * abs(norm)
     assert srv.stats()["windows"] >= 2
     srv.close()
+
+
+def test_graph_exec_updates_keep_results(gpu, oracle):
+    # Windows with the same classes and rounds take their new kernel
+    # arguments in the slot's existing graph exec instead of a new one; the
+    # finished norms of each window come back in one copy.  Every result is
+    # still the oracle's.
+    srv = Server(budget_bytes=1 << 30)
+    specs = [(0, 2, F32, 200 + 16 * i, 60, 1, 300 + i) for i in range(6)] + [(1, 3, F64, 150, 45, 1, 310)]
+    ids, res = run_all(srv, specs)
+    for rid, s in zip(ids, specs):
+        print(f"request {rid}: {s}", flush=True)
+        out, _, norm = ref_solve(oracle, s)
+        assert np.array_equal(srv.fetch(rid).view(np.uint8), out.view(np.uint8)), s
+        assert abs(res[rid]["norm"] - norm) <= 1e-13 * abs(norm), s
+    st = srv.stats()
+    print(st, flush=True)
+    assert st["windows"] >= 6 and st["graph_updates"] >= 1, st
+    srv.close()

@@ -86,4 +86,7 @@ def test_server_host_logic(tmp_path, sanitizer, args, gbs):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr[-4000:]
     assert "failures 0" in r.stdout
+    # windows whose topology repeats take the new arguments in their exec
+    updates = int(r.stdout.split("graph updates ")[-1].split(",")[0])
+    assert updates > 0
     assert "WARNING: ThreadSanitizer" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
