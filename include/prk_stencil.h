@@ -364,6 +364,14 @@ int prk_stream_mix(long long len, const double *in, double *in_next, double *out
  * blocks 0.  No reference counterpart. */
 int prk_copy_shape(long long bytes, void *dst, const void *src, int vpl, int nt, int blocks, void *stream);
 
+/* Diagnostics.  No counterpart in the reference; for processes that need to
+ * know where a host fault happened, such as a serving process whose worker
+ * thread dies.  enable != 0 installs handlers for SIGSEGV, SIGBUS, SIGILL,
+ * SIGFPE and SIGABRT that write the faulting thread's native frames to
+ * stderr and then pass the signal to the handler installed before them (or
+ * the default action); enable == 0 puts those handlers back. */
+int prk_crash_backtrace(int enable);
+
 /* ------------------------------------------------------------------------
  * Matrix transpose, the next bandwidth-bound kernel of the suite.  One
  * iteration is B += A^T and A += 1 on order x order FP64 row-major matrices

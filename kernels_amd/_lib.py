@@ -40,6 +40,7 @@ SIGNATURES = {
     "prk_triad": (c_int, [c_ll, c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "prk_copy": (c_int, [c_ll, c_void_p, c_void_p, c_void_p]),
     "prk_copy_shape": (c_int, [c_ll, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "prk_crash_backtrace": (c_int, [c_int]),
     "prk_stream_mix": (c_int, [c_ll, c_void_p, c_void_p, c_void_p, c_void_p]),
     "prk_transpose": (c_int, [c_int, c_int, c_void_p, c_void_p]),
     "prk_transpose_ex": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),

@@ -1,7 +1,10 @@
 // C ABI of libprkstencil.so: the drop-in operator table, the fused sweep,
 // initialisation, the device L1 norm, the driver loop and the HBM
 // calibration kernels.  Declarations and reference citations: prk_stencil.h.
+#include <execinfo.h>
 #include <hip/hip_runtime.h>
+#include <signal.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cstdarg>
@@ -424,6 +427,40 @@ int l1norm_async(int dtype, int n, long long pitch, int radius, const void *out,
     return enqueue_l1norm(dtype, n, pitch, radius, out, d_result, (DD *)partials, s);
 }
 
+// ---------------------------------------------------------------------------
+// Crash backtraces.  With the handlers installed, a fatal signal in any
+// thread (a host fault in the server's worker thread, say) first writes the
+// faulting thread's native frames to stderr, then the handler that was
+// there before (Python's faulthandler under pytest, or the default action)
+// runs.  Only async-signal-safe calls are made in the handler; backtrace()
+// is called once at install so that its first use, which may load libgcc,
+// is not in a handler.
+// ---------------------------------------------------------------------------
+constexpr int kCrashSignals[] = {SIGSEGV, SIGBUS, SIGILL, SIGFPE, SIGABRT};
+constexpr int kNumCrashSignals = (int)(sizeof(kCrashSignals) / sizeof(kCrashSignals[0]));
+struct sigaction g_prev_action[kNumCrashSignals];
+bool g_crash_installed = false;
+std::mutex g_crash_mu;
+
+void crash_write(const char *s) { (void)!write(2, s, strlen(s)); }
+
+void crash_handler(int sig)
+{
+    char digits[12], num[12];
+    int nd = 0, k = 0;
+    for (int v = sig; v > 0 || nd == 0; v /= 10) digits[nd++] = (char)('0' + v % 10);
+    while (nd) num[k++] = digits[--nd];
+    num[k] = 0;
+    crash_write("prk: fatal signal ");
+    crash_write(num);
+    crash_write(", native frames of the faulting thread:\n");
+    void *frames[64];
+    backtrace_symbols_fd(frames, backtrace(frames, 64), 2);
+    for (int i = 0; i < kNumCrashSignals; ++i)
+        if (kCrashSignals[i] == sig) sigaction(sig, &g_prev_action[i], nullptr);
+    raise(sig);  // delivered to the previous handler once this one returns
+}
+
 }  // namespace prk
 
 using namespace prk;
@@ -432,6 +469,29 @@ using namespace prk;
 // extern "C"
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int prk_crash_backtrace(int enable)
+{
+    std::lock_guard<std::mutex> lock(g_crash_mu);
+    if (enable && !g_crash_installed) {
+        void *f[1];
+        (void)backtrace(f, 1);
+        struct sigaction sa;
+        std::memset(&sa, 0, sizeof sa);
+        sa.sa_handler = crash_handler;
+        sigemptyset(&sa.sa_mask);
+        for (int i = 0; i < kNumCrashSignals; ++i)
+            if (sigaction(kCrashSignals[i], &sa, &g_prev_action[i]) != 0) {
+                for (int j = 0; j < i; ++j) sigaction(kCrashSignals[j], &g_prev_action[j], nullptr);
+                return set_error(PRK_EINVAL, "sigaction failed for signal %d", kCrashSignals[i]);
+            }
+        g_crash_installed = true;
+    } else if (!enable && g_crash_installed) {
+        for (int i = 0; i < kNumCrashSignals; ++i) sigaction(kCrashSignals[i], &g_prev_action[i], nullptr);
+        g_crash_installed = false;
+    }
+    return PRK_OK;
+}
 
 const char *prk_last_error(void) { return g_last_error.c_str(); }
 int prk_abi_version(void) { return PRK_STENCIL_ABI_VERSION; }

@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -13,6 +14,9 @@ _NP = {0: np.float64, 1: np.float32, 2: np.uint16}
 class Server:
     def __init__(self, budget_bytes: int, max_pending: int = 1024, max_active: int = 4096,
                  round_tiles: int = 16384):
+        if os.environ.get("PRK_CRASH_BACKTRACE"):
+            # a host fault (in the worker thread too) prints its native frames
+            check(lib().prk_crash_backtrace(1))
         cfg = ServerConfig(budget_bytes, max_pending, max_active, round_tiles)
         self.h = None
         h = ctypes.c_void_p()

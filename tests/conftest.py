@@ -25,5 +25,7 @@ def gpu():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     import kernels_amd
-    kernels_amd.lib()  # fail loudly if the library is missing
+    # fail loudly if the library is missing; a host fault during the GPU
+    # tests prints its native frames before pytest's Python traceback
+    kernels_amd.check(kernels_amd.lib().prk_crash_backtrace(1))
     return torch.device("cuda:0")

@@ -4,6 +4,7 @@ before touching the device."""
 import ctypes
 import re
 import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -186,3 +187,28 @@ def test_batch_tuning_table_defaults():
     assert L.prk_batch_tuning_get(0, 9, 1, 1000, ctypes.byref(r)) == -2   # radius 9: no stencil
     assert L.prk_batch_autotune(None, 0, 4, 1, None) == -1
 
+
+
+def test_crash_backtrace_on_host_fault():
+    # In a child process (host only, no device): with the handlers installed
+    # a host fault prints the faulting thread's native frames, then the
+    # signal reaches the previous handler (here the default action), so the
+    # process still dies of SIGSEGV.  Uninstalling puts the old handler back.
+    code = (
+        "import ctypes, signal, sys\n"
+        "import kernels_amd\n"
+        "L = kernels_amd.lib()\n"
+        "kernels_amd.check(L.prk_crash_backtrace(1))\n"
+        "kernels_amd.check(L.prk_crash_backtrace(0))\n"
+        "assert signal.getsignal(signal.SIGSEGV) == signal.SIG_DFL\n"
+        "kernels_amd.check(L.prk_crash_backtrace(1))\n"
+        "sys.stderr.write('about to fault\\n'); sys.stderr.flush()\n"
+        "ctypes.string_at(0)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120,
+                       cwd=str(Path(__file__).resolve().parents[1]))
+    assert r.returncode == -11, (r.returncode, r.stderr[-2000:])
+    assert "about to fault" in r.stderr
+    assert "prk: fatal signal 11, native frames of the faulting thread:" in r.stderr
+    frames = r.stderr.split("native frames of the faulting thread:")[1].strip().splitlines()
+    assert len(frames) >= 3 and any("libprkstencil" in f for f in frames), frames[:8]

@@ -76,10 +76,12 @@ def test_serving_client_loop(tmp_path):
                                                 ("address", ["100", "1", "200", "1"], 0),
                                                 ("thread", ["200", "1", "200", "0"], 0),
                                                 ("address", ["2000", "3", "200", "0"], 6000),
-                                                ("thread", ["500", "2", "200", "0"], 6000)])
+                                                ("thread", ["500", "2", "200", "0"], 6000),
+                                                ("undefined", ["8000", "2", "200", "0"], 6000)])
 def test_server_host_logic(tmp_path, sanitizer, args, gbs):
     exe = build(tmp_path, sanitizer)
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", TSAN_OPTIONS="halt_on_error=1")
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", TSAN_OPTIONS="halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
     if gbs:
         env["PRK_MOCK_GBS"] = str(gbs)
     r = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=300, env=env)
@@ -90,3 +92,4 @@ def test_server_host_logic(tmp_path, sanitizer, args, gbs):
     updates = int(r.stdout.split("graph updates ")[-1].split(",")[0])
     assert updates > 0
     assert "WARNING: ThreadSanitizer" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
+    assert "runtime error" not in r.stderr
