@@ -21,7 +21,17 @@
 // (tile_of): row-major over the grid measured best on MI355X; the orders
 // that give each of the eight XCDs a contiguous run of tiles, so that a
 // tile's vertical neighbours (whose rows it re-reads as halo) share its L2,
-// are kept as alternatives for the autotuner and for other shapes.
+// are kept as alternatives for the autotuner and for other shapes.  The
+// FP32/BF16 forms at R <= 3 may instead keep the widened windows of the last
+// 2R+1 rows in registers, neighbour columns arriving by DPP lane shifts
+// (wave_tile_rw, no LDS).
+//
+// Frame.  out's frame (its first and last R rows and columns) is never
+// written.  The one lane per edge strip and row whose vector straddles a
+// frame boundary parks its result in an LDS stash, and after the row loop
+// the wave writes those rows' interior elements (edge_store,
+// flush_edge_stash).  Grids the library owns (batched and served solves)
+// store the frame back with the bits it held instead (FRAME_REWRITE).
 //
 // Arithmetic.  FP64: the products are summed in the generator's order and
 // the sum is added to out last, with contraction off, so results are
