@@ -92,7 +92,8 @@ int prk_add(int n, int t, double *in);
 /* ------------------------------------------------------------------------
  * Fused sweep: out += S(in) on the interior and in_next = in + 1 on all
  * n*n points, in one pass (in_next must not alias in).  Rows are `pitch`
- * elements apart (pitch >= n); columns n..pitch-1 are never written.
+ * elements apart (pitch >= n); columns n..pitch-1 are never written, nor
+ * is the frame of out (its first and last R rows and columns).
  * With pitch*size and the pointers 16-byte aligned the kernel moves whole
  * 16-byte vectors; otherwise it falls back to element accesses.
  * FP64 results are bitwise those of the unfused drop-in pair
