@@ -750,7 +750,7 @@ def fp32_storage_section(a, torch, K, ms_f64: float) -> dict:
     K.run(pattern, radius, K.F64, iters, b64[0], b64[1], y, stream)
     del b64
     torch.cuda.empty_cache()
-    tot = {"sum_abs": 0.0, "sum_rel": 0.0, "max_abs": 0.0, "max_rel": 0.0, "d2": 0.0, "y2": 0.0}
+    tot = {"sum_abs": 0.0, "sum_rel": 0.0, "max_abs": 0.0, "max_rel": 0.0, "d2": 0.0, "y2": 0.0, "l1x": 0.0, "l1y": 0.0}
     inner = slice(radius, n - radius)
     tau = 1e-3 * float(y[inner, inner].abs().mean())
     for r0 in range(radius, n - radius, 2048):
@@ -761,6 +761,7 @@ def fp32_storage_section(a, torch, K, ms_f64: float) -> dict:
         tot["sum_abs"] += float(d.sum()); tot["sum_rel"] += float(rel.sum())
         tot["max_abs"] = max(tot["max_abs"], float(d.max())); tot["max_rel"] = max(tot["max_rel"], float(rel.max()))
         tot["d2"] += float((d * d).sum()); tot["y2"] += float((yy * yy).sum())
+        tot["l1x"] += float(out[r0:r1, inner].double().abs().sum()); tot["l1y"] += float(yy.abs().sum())
     pts = (n - 2 * radius) ** 2
     normwise = (tot["d2"] / tot["y2"]) ** 0.5
     del out, y
@@ -777,6 +778,8 @@ def fp32_storage_section(a, torch, K, ms_f64: float) -> dict:
                                 "mean_abs": float(f"{tot['sum_abs'] / pts:.3e}"),
                                 "max_rel": float(f"{tot['max_rel']:.3e}"),
                                 "mean_rel": float(f"{tot['sum_rel'] / pts:.3e}"),
+                                # the validated quantity: the interior L1 norm, FP32 against FP64
+                                "l1_norm_rel": float(f"{abs(tot['l1x'] - tot['l1y']) / tot['l1y']:.3e}"),
                                 "target_normwise": 1e-6, "target_met": bool(normwise <= 1e-6)},
     }
 

@@ -55,6 +55,7 @@ def test_default_workload_line(gpu):
     assert f32["validated"] is True and f32["points_per_s_vs_f64"] > 1.0
     err = f32["error_vs_f64_seeded"]
     assert 0 < err["normwise_l2"] < 1e-5 and err["target_normwise"] == 1e-6
+    assert 0 <= err["l1_norm_rel"] < 1e-6
     bat = d["batched_config4"]
     assert bat["solves_per_s"] > bat["one_by_one_f64_solves_per_s"] > 0
     assert bat["max_rel_norm_diff_vs_f64"] < 1e-5
