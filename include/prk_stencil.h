@@ -310,6 +310,9 @@ typedef struct prk_server_config {
     int max_pending;        /* queued (not yet admitted) requests beyond which submit refuses */
     int max_active;         /* requests in flight at most */
     long long round_tiles;  /* admit while the active requests' tiles per round stay below this */
+    int shadow_every;       /* > 0: every shadow_every-th FP32/BF16 request of each (dtype, pattern,
+                               radius) class also runs in FP64, and its result carries its error against
+                               that FP64 solve; 0: none */
 } prk_server_config;
 
 typedef struct prk_server_result {
@@ -317,6 +320,12 @@ typedef struct prk_server_result {
     double norm;
     double t_submit, t_admit, t_done;
     int status;
+    /* Shadowed requests (see shadow_every): err_valid 1 and the error of out
+     * against the same solve in FP64 over the interior -- max |x-y|, mean
+     * |x-y| and the normwise sqrt(sum (x-y)^2 / sum y^2).  t_done is then
+     * when both solves had finished. */
+    int err_valid;
+    double err_max_abs, err_mean_abs, err_normwise;
 } prk_server_result;
 
 typedef struct prk_server_stats {
@@ -327,6 +336,7 @@ typedef struct prk_server_stats {
     double build_seconds; /* host time spent building and capturing windows */
     double wait_seconds;  /* host time spent waiting for windows to finish */
     long long graph_updates; /* windows whose graph exec took new arguments in place instead of being rebuilt */
+    long long shadow_solves; /* requests that got an FP64 shadow solve */
 } prk_server_stats;
 
 int prk_server_create(const prk_server_config *cfg, prk_server **out);

@@ -94,18 +94,20 @@ SIGNATURES.update({
 
 
 class ServerConfig(ctypes.Structure):
-    _fields_ = [("budget_bytes", c_ll), ("max_pending", c_int), ("max_active", c_int), ("round_tiles", c_ll)]
+    _fields_ = [("budget_bytes", c_ll), ("max_pending", c_int), ("max_active", c_int), ("round_tiles", c_ll),
+                ("shadow_every", c_int)]
 
 
 class ServerResult(ctypes.Structure):
     _fields_ = [("id", c_u64), ("norm", c_double), ("t_submit", c_double), ("t_admit", c_double),
-                ("t_done", c_double), ("status", c_int)]
+                ("t_done", c_double), ("status", c_int), ("err_valid", c_int), ("err_max_abs", c_double),
+                ("err_mean_abs", c_double), ("err_normwise", c_double)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(f, c_ll) for f in ("submitted", "served", "rejected", "windows", "launches", "pending", "active",
                                     "pool_used", "budget")] + [("build_seconds", c_double), ("wait_seconds", c_double),
-                                                               ("graph_updates", c_ll)]
+                                                               ("graph_updates", c_ll), ("shadow_solves", c_ll)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

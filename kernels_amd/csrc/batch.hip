@@ -183,6 +183,69 @@ namespace prk {
 
 int batch_class_of(const prk_solve &s) { return (s.dtype * 2 + s.pattern) * kBatchMaxRadius + (s.radius - 1); }
 
+// Fixed-order block reduction of four values (max, then three sums).
+__device__ inline void err_block_reduce(double (&v)[4], double *out)
+{
+    __shared__ double red[4][256];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + w]);
+#pragma unroll
+            for (int k = 1; k < 4; ++k) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void error_partials_kernel(const void *x_, long long xp, const double *y,
+                                                             long long yp, int n, int R, double *partials)
+{
+    typedef typename DTypeTraits<DT>::T T;
+    const T *x = (const T *)x_;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = R + (int)blockIdx.x; i < n - R; i += (int)gridDim.x)
+        for (int j = R + (int)threadIdx.x; j < n - R; j += (int)blockDim.x) {
+            const double yv = y[(long long)i * yp + j];
+            const double d = fabs(to_f32<T>(x[(long long)i * xp + j]) - yv);
+            v[0] = fmax(v[0], d);
+            v[1] += d;
+            v[2] += d * d;
+            v[3] += yv * yv;
+        }
+    err_block_reduce(v, partials + 4 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void error_final_kernel(const double *partials, double *out)
+{
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = (int)threadIdx.x; b < kErrBlocks; b += (int)blockDim.x) {
+        v[0] = fmax(v[0], partials[4 * b]);
+#pragma unroll
+        for (int k = 1; k < 4; ++k) v[k] += partials[4 * b + k];
+    }
+    err_block_reduce(v, out);
+}
+
+int error_vs_f64_launch(int dtype, const void *x, long long xp, const double *y, long long yp, int n, int R,
+                        double *partials, double *out, hipStream_t s)
+{
+    if (dtype == F32)
+        hipLaunchKernelGGL(error_partials_kernel<F32>, dim3(kErrBlocks), dim3(256), 0, s, x, xp, y, yp, n, R,
+                           partials);
+    else if (dtype == BF16)
+        hipLaunchKernelGGL(error_partials_kernel<BF16>, dim3(kErrBlocks), dim3(256), 0, s, x, xp, y, yp, n, R,
+                           partials);
+    else
+        return set_error(PRK_EINVAL, "error against FP64 needs an FP32 or BF16 result");
+    hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, partials, out);
+    return check_hip(hipGetLastError(), "error kernel launch");
+}
+
 int batch_launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s)
 {
     const int radius = cls % kBatchMaxRadius + 1;

@@ -210,4 +210,13 @@ constexpr int kBatchClasses = 3 * 2 * kBatchMaxRadius;
 int batch_launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t s);
 int batch_validate_solve(const prk_solve &s);
 
+// Error of a reduced-precision result x (FP32 or BF16, pitch xp) against the
+// same solve in FP64 y (pitch yp) over the interior R <= i, j < n-R:
+// out[0] = max |x-y|, out[1] = sum |x-y|, out[2] = sum (x-y)^2,
+// out[3] = sum y^2, each summed in a fixed order (kErrBlocks partials of
+// four doubles in `partials`, then one block), on stream s.
+constexpr int kErrBlocks = 256;
+int error_vs_f64_launch(int dtype, const void *x, long long xp, const double *y, long long yp, int n, int R,
+                        double *partials, double *out, hipStream_t s);
+
 }  // namespace prk

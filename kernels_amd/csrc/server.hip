@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -97,6 +98,13 @@ struct Req {
     double norm = 0;
     bool done = false;
     int pins = 0;  // fetches in progress (release waits for them)
+    // FP64 shadow solve: `twin` of a reduced-precision request, `owner` of
+    // the internal twin; the owner is answered once both have finished, with
+    // its error against the twin
+    Req *twin = nullptr, *owner = nullptr;
+    bool internal = false, finished = false;
+    int err_valid = 0;
+    double err_max_abs = 0, err_mean_abs = 0, err_normwise = 0;
 };
 
 constexpr int kRoundsPerWindow = 8;
@@ -122,12 +130,20 @@ struct prk_server {
     size_t h_tab_cap[2] = {0, 0};
     size_t norm_off[2] = {0, 0};  // finishing requests' norms in the slot's tables
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    // FP64 shadow solves: error kernels on their own stream, partials and the
+    // result on the device, the result in pinned memory, a count per class
+    hipStream_t aux = nullptr;
+    hipEvent_t err_done = nullptr;
+    double *d_err = nullptr;  // kErrBlocks * 4 partials, then 4 results
+    double *h_err = nullptr;
+    long long shadow_seq[kBatchClasses] = {};
 
     std::mutex mu;
     std::condition_variable cv_work, cv_idle;
     std::deque<Req *> pending;
     std::vector<Req *> active;  // admitted, still has stages to schedule
     std::vector<Req *> inflight[2];  // finishing in the window of that slot
+    std::vector<Req *> awaiting;     // finished, waiting for their shadow partner
     std::unordered_map<uint64_t, Req *> kept;
     std::vector<prk_server_result> completed;
     uint64_t next_id = 1;
@@ -202,6 +218,54 @@ int ensure_capacity(prk_server *s, int slot, size_t bytes)
     return PRK_OK;
 }
 
+// Answer a finished user request (lock held): its result, then its memory
+// kept for fetches or returned to the pool.
+void complete_locked(prk_server *s, Req *r)
+{
+    prk_server_result res;
+    std::memset(&res, 0, sizeof res);
+    res.id = r->id;
+    res.norm = r->norm;
+    res.t_submit = r->t_submit;
+    res.t_admit = r->t_admit;
+    res.t_done = r->t_done;
+    res.status = PRK_OK;
+    res.err_valid = r->err_valid;
+    res.err_max_abs = r->err_max_abs;
+    res.err_mean_abs = r->err_mean_abs;
+    res.err_normwise = r->err_normwise;
+    s->completed.push_back(res);
+    s->st.served += 1;
+    if (r->keep) {
+        s->kept[r->id] = r;
+    } else {
+        s->pool.release(r->off, r->bytes);
+        delete r;
+    }
+}
+
+// Error of a shadowed request against its FP64 twin, both finished: a
+// fixed-order reduction on the error stream (the window stream may already
+// run the next window; these grids are no longer written).
+int shadow_error(prk_server *s, Req *r, Req *t)
+{
+    const int wd = r->g.n - 2 * r->g.radius;
+    int rc = error_vs_f64_launch(r->s.dtype, r->g.out, r->g.pitch, (const double *)t->g.out, t->g.pitch, r->g.n,
+                                 r->g.radius, s->d_err, s->d_err + 4 * kErrBlocks, s->aux);
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(s->h_err, s->d_err + 4 * kErrBlocks, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                                      s->aux),
+                       "server error copy");
+    if (!rc) rc = check_hip(hipEventRecord(s->err_done, s->aux), "server error event");
+    if (!rc) rc = check_hip(hipEventSynchronize(s->err_done), "server error wait");
+    if (rc) return rc;
+    r->err_valid = 1;
+    r->err_max_abs = s->h_err[0];
+    r->err_mean_abs = wd > 0 ? s->h_err[1] / ((double)wd * (double)wd) : 0.0;
+    r->err_normwise = s->h_err[3] > 0 ? std::sqrt(s->h_err[2] / s->h_err[3]) : 0.0;
+    return PRK_OK;
+}
+
 // Retire the requests whose final stage ran in the window of `slot`.
 void retire(prk_server *s, int slot)
 {
@@ -220,24 +284,39 @@ void retire(prk_server *s, int slot)
         r->t_done = t;
         r->done = true;
     }
-    std::lock_guard<std::mutex> lock(s->mu);
-    for (Req *r : fin) {
-        prk_server_result res;
-        res.id = r->id;
-        res.norm = r->norm;
-        res.t_submit = r->t_submit;
-        res.t_admit = r->t_admit;
-        res.t_done = r->t_done;
-        res.status = PRK_OK;
-        s->completed.push_back(res);
-        s->st.served += 1;
-        if (r->keep) {
-            s->kept[r->id] = r;
-        } else {
-            s->pool.release(r->off, r->bytes);
-            delete r;
+    // shadowed requests and their twins pair up; a pair is compared once
+    // both have finished
+    std::vector<Req *> pairs;  // owners whose twin has finished too
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        for (Req *r : fin) {
+            Req *owner = r->internal ? r->owner : r;
+            if (!owner->twin) {
+                complete_locked(s, r);
+                continue;
+            }
+            r->finished = true;
+            Req *other = r->internal ? owner : owner->twin;
+            if (other->finished) {
+                s->awaiting.erase(std::remove(s->awaiting.begin(), s->awaiting.end(), other), s->awaiting.end());
+                pairs.push_back(owner);
+            } else {
+                s->awaiting.push_back(r);
+            }
         }
     }
+    for (Req *r : pairs) {
+        Req *tw = r->twin;
+        const int rc = shadow_error(s, r, tw);
+        if (rc) fail(s, rc);
+        std::lock_guard<std::mutex> lock(s->mu);
+        r->t_done = std::max(r->t_done, tw->t_done);
+        r->twin = nullptr;
+        s->pool.release(tw->off, tw->bytes);
+        delete tw;
+        complete_locked(s, r);
+    }
+    std::lock_guard<std::mutex> lock(s->mu);
     s->st.pool_used = (long long)s->pool.used();
     s->cv_idle.notify_all();
 }
@@ -473,13 +552,19 @@ void server_free(prk_server *s)
     if (s->capture) (void)hipStreamDestroy(s->capture);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->pool_base) (void)hipFree(s->pool_base);
-    // A request is in exactly one of these: queued, active, finishing in a
-    // window that was never retired (the worker stopped on an error), kept.
+    // A request (or shadow twin) is in exactly one of these: queued, active,
+    // finishing in a window that was never retired (the worker stopped on an
+    // error), finished and waiting for its shadow partner, kept.
     for (Req *r : s->pending) delete r;
     for (Req *r : s->active) delete r;
     for (auto &v : s->inflight)
         for (Req *r : v) delete r;
+    for (Req *r : s->awaiting) delete r;
     for (auto &kv : s->kept) delete kv.second;
+    if (s->err_done) (void)hipEventDestroy(s->err_done);
+    if (s->aux) (void)hipStreamDestroy(s->aux);
+    if (s->d_err) (void)hipFree(s->d_err);
+    if (s->h_err) (void)hipHostFree(s->h_err);
     delete s;
 }
 
@@ -490,7 +575,8 @@ extern "C" {
 int prk_server_create(const prk_server_config *cfg, prk_server **out)
 {
     if (!cfg || !out) return set_error(PRK_EINVAL, "null server config");
-    if (cfg->budget_bytes < (1 << 20) || cfg->max_pending < 1 || cfg->max_active < 1 || cfg->round_tiles < 1)
+    if (cfg->budget_bytes < (1 << 20) || cfg->max_pending < 1 || cfg->max_active < 1 || cfg->round_tiles < 1 ||
+        cfg->shadow_every < 0)
         return set_error(PRK_EINVAL, "bad server config");
     prk_server *s = new prk_server();
     s->cfg = *cfg;
@@ -511,6 +597,14 @@ int prk_server_create(const prk_server_config *cfg, prk_server **out)
     }
     for (int k = 0; k < 2 && !rc; ++k)
         rc = check_hip(hipEventCreateWithFlags(&s->window_done[k], hipEventDisableTiming), "server event");
+    if (!rc && cfg->shadow_every > 0) {
+        rc = check_hip(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking), "server error stream");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&s->err_done, hipEventDisableTiming), "server event");
+        if (!rc)
+            rc = check_hip(hipMalloc((void **)&s->d_err, sizeof(double) * (4 * kErrBlocks + 4)), "server error buffer");
+        if (!rc) rc = check_hip(hipHostMalloc((void **)&s->h_err, 4 * sizeof(double), hipHostMallocDefault),
+                                "server error staging");
+    }
     if (rc) {
         server_free(s);
         return rc;
@@ -548,6 +642,25 @@ int prk_server_submit(prk_server *s, const prk_solve *solve, int keep_output, ui
     r->id = s->next_id++;
     s->pending.push_back(r);
     s->st.submitted += 1;
+    // every shadow_every-th reduced-precision request of its class gets an
+    // internal FP64 twin (queued right behind it) to measure its error
+    if (s->cfg.shadow_every > 0 && solve->dtype != PRK_F64 &&
+        s->shadow_seq[batch_class_of(*solve)]++ % s->cfg.shadow_every == 0) {
+        Req *t = new Req();
+        t->s = *solve;
+        t->s.dtype = PRK_F64;
+        t->bytes = request_bytes(t->s, t->g);
+        if ((long long)(t->bytes + r->bytes) <= s->cfg.budget_bytes) {
+            t->internal = true;
+            t->owner = r;
+            t->t_submit = r->t_submit;
+            r->twin = t;
+            s->pending.push_back(t);
+            s->st.shadow_solves += 1;
+        } else {
+            delete t;  // the pair could never be resident at once
+        }
+    }
     if (id) *id = r->id;
     s->cv_work.notify_one();
     return PRK_OK;

@@ -13,11 +13,11 @@ _NP = {0: np.float64, 1: np.float32, 2: np.uint16}
 
 class Server:
     def __init__(self, budget_bytes: int, max_pending: int = 1024, max_active: int = 4096,
-                 round_tiles: int = 16384):
+                 round_tiles: int = 16384, shadow_every: int = 0):
         if os.environ.get("PRK_CRASH_BACKTRACE"):
             # a host fault (in the worker thread too) prints its native frames
             check(lib().prk_crash_backtrace(1))
-        cfg = ServerConfig(budget_bytes, max_pending, max_active, round_tiles)
+        cfg = ServerConfig(budget_bytes, max_pending, max_active, round_tiles, shadow_every)
         self.h = None
         h = ctypes.c_void_p()
         check(lib().prk_server_create(ctypes.byref(cfg), ctypes.byref(h)))
@@ -47,8 +47,14 @@ class Server:
         buf = (ServerResult * max_results)()
         n = ctypes.c_int()
         check(lib().prk_server_poll(self.h, buf, max_results, ctypes.byref(n)))
-        return [{"id": r.id, "norm": r.norm, "t_submit": r.t_submit, "t_admit": r.t_admit, "t_done": r.t_done}
-                for r in buf[: n.value]]
+        out = []
+        for r in buf[: n.value]:
+            d = {"id": r.id, "norm": r.norm, "t_submit": r.t_submit, "t_admit": r.t_admit, "t_done": r.t_done}
+            if r.err_valid:  # shadowed: error against the same solve in FP64
+                d["error_vs_f64"] = {"max_abs": r.err_max_abs, "mean_abs": r.err_mean_abs,
+                                     "normwise": r.err_normwise}
+            out.append(d)
+        return out
 
     def drain(self, timeout_s: float = 60.0) -> None:
         check(lib().prk_server_drain(self.h, timeout_s))

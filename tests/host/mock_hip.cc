@@ -137,6 +137,7 @@ static void check_idle_locked(const void *p, size_t bytes, const char *op)
 // Failure injection: the instantiation with this ordinal (1-based) fails.
 long long g_instantiations = 0, g_fail_instantiation = 0;
 long long g_updates = 0;  // exec updates in place
+long long g_error_launches = 0;  // shadow comparisons
 
 #define MOCK_CHECK(c, ...)                                                            \
     do {                                                                              \
@@ -194,6 +195,24 @@ static double replay(const Launch &l, std::vector<const void *> &norms)
         }
     }
     return bytes;
+}
+
+// The error of a shadowed request against its FP64 twin: the stand-in checks
+// that both grids and the buffers lie in live device memory and reports no
+// error (its replay does not compute the grids).
+int error_vs_f64_launch(int dtype, const void *x, long long xp, const double *y, long long yp, int n, int R,
+                        double *partials, double *out, hipStream_t)
+{
+    const size_t es = dtype == 1 ? 4 : 2;
+    MOCK_CHECK(dtype == 1 || dtype == 2, "error of a dtype %d result", dtype);
+    MOCK_CHECK(R >= 1 && n > 2 * R && xp >= n && yp >= n, "error shape");
+    MOCK_CHECK(is_dev(x, es * (size_t)n * (size_t)xp) && is_dev(y, 8 * (size_t)n * (size_t)yp),
+               "error grids outside the pool");
+    MOCK_CHECK(is_dev(partials, sizeof(double) * 4 * kErrBlocks) && is_dev(out, 4 * sizeof(double)),
+               "error buffers outside device memory");
+    for (int k = 0; k < 4; ++k) out[k] = k == 3 ? 1.0 : 0.0;
+    ++g_error_launches;
+    return 0;
 }
 
 int batch_launch_round(int cls, const RoundArgs &a, unsigned blocks, hipStream_t)

@@ -25,7 +25,7 @@
 namespace prk {
 extern long long g_items, g_final;
 extern int g_check_all;
-extern long long g_instantiations, g_fail_instantiation, g_updates;
+extern long long g_instantiations, g_fail_instantiation, g_updates, g_error_launches;
 }  // namespace prk
 
 static prk_solve random_solve(std::mt19937_64 &rng, int nmin, int nmax, int rmax = 4)
@@ -207,7 +207,52 @@ int main(int argc, char **argv)
         prk_server_destroy(s);
         prk::g_fail_instantiation = 0;
     }
-    std::printf("entries %lld, final norms %lld, graph updates %lld, failures %d\n", prk::g_items, prk::g_final,
-                prk::g_updates, fails);
+    // phase 5: every second FP32/BF16 request of each class also runs in FP64
+    // (an internal twin): every request is answered once, the shadowed ones
+    // with an error and none of the FP64 ones, the twins never, and the pool
+    // is empty once the kept outputs are released
+    {
+        std::mt19937_64 rng(19);
+        prk_server_config cfg{2ll << 30, 256, 4096, 16384, 2};
+        prk_server *s = nullptr;
+        if (prk_server_create(&cfg, &s)) return 1;
+        std::map<uint64_t, prk_solve> ids;
+        for (int i = 0; i < 300; ++i) {
+            const prk_solve sv = random_solve(rng, 16, 400);
+            uint64_t id = 0;
+            const int rc = prk_server_submit(s, &sv, i % 5 == 0, &id);
+            if (rc == PRK_OK) ids[id] = sv;
+            else fails += rc != PRK_EBUSY;
+            if (i % 25 == 0) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+        fails += prk_server_drain(s, 600) != PRK_OK;
+        std::vector<prk_server_result> res(ids.size() + 16);
+        int cnt = 0, with_err = 0, bad = 0;
+        fails += prk_server_poll(s, res.data(), (int)res.size(), &cnt) != PRK_OK;
+        std::map<uint64_t, int> seen;
+        for (int i = 0; i < cnt; ++i) {
+            auto it = ids.find(res[i].id);
+            if (it == ids.end() || seen[res[i].id]++ || res[i].norm != 2.0 * (it->second.iterations + 1)) {
+                ++bad;
+                continue;
+            }
+            if (res[i].err_valid) {
+                ++with_err;
+                bad += it->second.dtype == PRK_F64 || res[i].t_done < res[i].t_admit;
+            }
+        }
+        prk_server_stats st{};
+        prk_server_stats_get(s, &st);
+        for (auto &kv : ids) (void)prk_server_release(s, kv.first);  // the kept ones
+        prk_server_stats end{};
+        prk_server_stats_get(s, &end);
+        std::printf("shadow solves: accepted %zu, answered %d, wrong %d, with error %d, shadowed %lld, "
+                    "pool in use %lld\n", ids.size(), cnt, bad, with_err, st.shadow_solves, end.pool_used);
+        fails += bad || cnt != (int)ids.size() || with_err != st.shadow_solves || st.shadow_solves == 0 ||
+                 end.pool_used != 0;
+        prk_server_destroy(s);
+    }
+    std::printf("entries %lld, final norms %lld, graph updates %lld, shadow comparisons %lld, failures %d\n",
+                prk::g_items, prk::g_final, prk::g_updates, prk::g_error_launches, fails);
     return fails ? 1 : 0;
 }

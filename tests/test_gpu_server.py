@@ -128,3 +128,36 @@ def test_graph_exec_updates_keep_results(gpu, oracle):
     print(st, flush=True)
     assert st["windows"] >= 6 and st["graph_updates"] >= 1, st
     srv.close()
+
+
+def test_shadow_solves_report_error_vs_f64(gpu, oracle):
+    # With shadow_every=1 every FP32/BF16 request also runs in FP64 inside
+    # the server; its result carries the error of out against that solve.
+    # Both solves are bitwise the oracle's (FP64 and the FP32/BF16
+    # restatement), so the error must be the oracle outputs' error: the max
+    # exactly, the sums up to their order.
+    from tests._oracle import bf16_bits_to_f64
+    specs = [(0, 2, F32, 130, 5, 1, 41), (1, 3, BF16, 96, 4, 1, 42), (1, 1, F32, 300, 7, 1, 43),
+             (0, 4, BF16, 257, 3, 1, 44), (1, 2, F64, 120, 3, 1, 45)]
+    srv = Server(budget_bytes=1 << 30, shadow_every=1)
+    ids, res = run_all(srv, specs)
+    assert srv.stats()["shadow_solves"] == 4
+    for rid, s in zip(ids, specs):
+        print(f"request {rid}: {s}", flush=True)
+        r = res[rid]
+        pattern, radius, dtype, n, iters, init, seed = s
+        if dtype == F64:
+            assert "error_vs_f64" not in r
+            continue
+        x, _ = oracle.run_lp(dtype, pattern, radius, n, iters, init, seed)
+        x = bf16_bits_to_f64(x) if dtype == BF16 else x.astype(np.float64)
+        y, _ = oracle.run(pattern, radius, n, 32, iters, init, seed)
+        inner = (slice(radius, n - radius),) * 2
+        d = np.abs(x[inner] - y[inner])
+        got = r["error_vs_f64"]
+        assert got["max_abs"] == d.max(), (s, got)
+        assert abs(got["mean_abs"] - d.mean()) <= 1e-12 * d.mean(), (s, got)
+        nw = float(np.sqrt((d * d).sum() / (y[inner] ** 2).sum()))
+        assert abs(got["normwise"] - nw) <= 1e-12 * nw, (s, got, nw)
+        assert r["t_admit"] <= r["t_done"]
+    srv.close()
