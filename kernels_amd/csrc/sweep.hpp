@@ -392,6 +392,87 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
     if constexpr (A < R) pair_sum_from<T, P, R, V, HP, A + 1>(rows, acc);
 }
 
+// ---------------------------------------------------------------------------
+// Grid stencils in the reduced-precision forms: ring sums.  Ring j of a grid
+// stencil (the offsets at Chebyshev distance j) has one weight on its edges,
+// +ce_j on column +j and on row +j and -ce_j on column -j and on row -j (the
+// corners excluded), and +-cc_j on the corners (j, j) and (-j, -j)
+// (weight_ab).  So, x(a, b) the input at row offset a, column offset b,
+//   S = sum_j ce_j * (H_j + V_j) + cc_j * C_j,
+//   V_j = VS_{j-1}(j) - VS_{j-1}(-j), VS_k(b) the sum of column b over rows
+//         -k..k: VS_0(b) = x(0, b), VS_k(b) = VS_{k-1}(b) + (x(k,b) + x(-k,b));
+//   H_j = D_j(0) + D_j(-1) + D_j(1) + ... + D_j(-(j-1)) + D_j(j-1), left to
+//         right, D_j(b) = x(j, b) - x(-j, b);
+//   C_j = x(j, j) - x(-j, -j);
+//   acc = ce_1 * (H_1 + V_1), then fmaf(cc_j, C_j, acc) and
+//   fmaf(ce_{j+1}, H_{j+1} + V_{j+1}, acc) in ring order.
+// The column sums VS and the differences D are formed once per column and
+// shared by the lane's V points.  Per point at R=3 that is 29 FP32
+// operations (BF16, V=8) or 32 (FP32, V=4) against the 42 of the paired
+// form, whose every tap pair costs a subtraction and an fma: the BF16 grid
+// sweeps issue VALU instructions near the CUs' rate, the HBM-bound ones
+// do not notice.
+// ---------------------------------------------------------------------------
+template <int R, int J>
+constexpr float ring_edge_w() { return (float)weight_ab(GRID, R, 0, J); }   // tap (J, 0)
+template <int R, int J>
+constexpr float ring_corner_w() { return (float)weight_ab(GRID, R, J, J); } // tap (J, J)
+
+// Ring J for the lane's V points from the widened windows of rows +J (wp)
+// and -J (wm); vs holds VS_{J-1} on entry and VS_J on return.  Window index
+// HP + v + b is column b of point v.
+template <int R, int V, int HP, int J>
+__device__ inline void ring_step(const float (&wp)[V + 2 * HP], const float (&wm)[V + 2 * HP],
+                                 float (&vs)[V + 2 * HP], float (&acc)[V])
+{
+#pragma clang fp contract(off)
+    static_assert(HP >= R, "the window holds columns -R..V-1+R");
+    constexpr float ce = ring_edge_w<R, J>(), cc = ring_corner_w<R, J>();
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int c = HP + v;
+        const float vj = vs[c + J] - vs[c - J];
+        float hj = wp[c] - wm[c];
+#pragma unroll
+        for (int m = 1; m < J; ++m) {
+            hj = hj + (wp[c - m] - wm[c - m]);
+            hj = hj + (wp[c + m] - wm[c + m]);
+        }
+        const float tj = hj + vj;
+        const float cj = wp[c + J] - wm[c - J];
+        if constexpr (J == 1)
+            acc[v] = ce * tj;
+        else
+            acc[v] = __builtin_fmaf(ce, tj, acc[v]);
+        acc[v] = __builtin_fmaf(cc, cj, acc[v]);
+    }
+    if constexpr (J < R) {
+#pragma unroll
+        for (int i = HP - R; i < HP + V + R; ++i) vs[i] = vs[i] + (wp[i] + wm[i]);
+    }
+}
+
+template <typename T, int R, int V, int HP, int J>
+__device__ inline void ring_sum_from(const T *const (&rows)[2 * R + 1], float (&vs)[V + 2 * HP], float (&acc)[V])
+{
+    constexpr bool whole = sizeof(T) == 2 && (R == 5 || R == 7);
+    float wp[V + 2 * HP], wm[V + 2 * HP];
+    load_window<T, V, HP, R, whole>(rows[R + J], wp);
+    load_window<T, V, HP, R, whole>(rows[R - J], wm);
+    ring_step<R, V, HP, J>(wp, wm, vs, acc);
+    if constexpr (J < R) ring_sum_from<T, R, V, HP, J + 1>(rows, vs, acc);
+}
+
+// The grid sum from the LDS ring (rows as in compute_points).
+template <typename T, int R, int V, int HP>
+__device__ inline void ring_sum(const T *const (&rows)[2 * R + 1], float (&acc)[V])
+{
+    constexpr bool whole = sizeof(T) == 2 && (R == 5 || R == 7);
+    float vs[V + 2 * HP];
+    load_window<T, V, HP, R, whole>(rows[R], vs);
+    ring_sum_from<T, R, V, HP, 1>(rows, vs, acc);
+}
+
 // The V outputs of one lane for output row o.  rows[k] points at the LDS row
 // holding input row o-R+k, already offset to this lane's first column.
 template <int DT, int P, int R, int V, int HP>
@@ -416,7 +497,10 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
         for (int v = 0; v < V; ++v) res[v] = outv[v] + sum[v];
     } else {
         float acc[V];
-        pair_sum_from<T, P, R, V, HP, 0>(rows, acc);
+        if constexpr (P == GRID)
+            ring_sum<T, R, V, HP>(rows, acc);
+        else
+            pair_sum_from<T, P, R, V, HP, 0>(rows, acc);
         float f[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) f[v] = to_f32<T>(outv[v]) + acc[v];
@@ -1123,6 +1207,27 @@ __device__ inline void rw_pair_sum(const Slot (&W)[S], int k, float (&acc)[V])
     if constexpr (A < R) rw_pair_sum<P, R, V, S, A + 1>(W, k, acc);
 }
 
+// Grid ring sums over the ring slots (rows as in rw_pair_sum).
+template <int R, int V, int S, int J, typename Slot>
+__device__ inline void rw_ring_from(const Slot (&W)[S], int k, float (&vs)[V + 2 * R], float (&acc)[V])
+{
+    ring_step<R, V, R, J>(W[(k + 1 + R + J) % S].w, W[(k + 1 + R - J) % S].w, vs, acc);
+    if constexpr (J < R) rw_ring_from<R, V, S, J + 1>(W, k, vs, acc);
+}
+
+template <int P, int R, int V, int S, typename Slot>
+__device__ inline void rw_sum(const Slot (&W)[S], int k, float (&acc)[V])
+{
+    if constexpr (P == GRID) {
+        float vs[V + 2 * R];
+#pragma unroll
+        for (int i = 0; i < V + 2 * R; ++i) vs[i] = W[(k + 1 + R) % S].w[i];
+        rw_ring_from<R, V, S, 1>(W, k, vs, acc);
+    } else {
+        rw_pair_sum<P, R, V, S, 0>(W, k, acc);
+    }
+}
+
 template <int DT, int P, int R, bool FUSE, int MODE, int PF, int NT>
 __device__ inline void wave_tile_rw(const void *in_, void *in_next_, void *out_, long long pitch, int n,
                                     const RowSpan &sp, int c0, int r0, int r1, int pad_writable, int lane,
@@ -1217,7 +1322,7 @@ __device__ inline void wave_tile_rw(const void *in_, void *in_next_, void *out_,
             const int o = ri - R;
             if (o >= olo && o < ohi) {
                 float acc[V];
-                rw_pair_sum<P, R, V, S, 0>(W, k, acc);
+                rw_sum<P, R, V, S>(W, k, acc);
                 float f[V];
 #pragma unroll
                 for (int v = 0; v < V; ++v) f[v] = to_f32<T>(cur.outv.e[v]) + acc[v];

@@ -192,8 +192,9 @@ template <> inline uint16_t st<uint16_t>(float x) { return f32_to_bf16(x); }
 
 struct Pair { int di, dj; float w; };
 
-// Taps of positive weight, weights rounded to float, ordered by row-offset
-// magnitude |di| = 0..R and in generator order within one magnitude.
+// Star stencils: the taps of positive weight, weights rounded to float,
+// ordered by row-offset magnitude |di| = 0..R and in generator order within
+// one magnitude.
 std::vector<Pair> make_pairs(int pattern, int R)
 {
     const TapList L = make_taps(pattern, R);
@@ -205,20 +206,72 @@ std::vector<Pair> make_pairs(int pattern, int R)
     return v;
 }
 
+// The reduced-precision operator.  x(a, b) is the input at row offset a,
+// column offset b from the point, widened to float.
+struct LpOp {
+    int pattern, R;
+    std::vector<Pair> pairs;  // star
+    float ce[max_radius + 1], cc[max_radius + 1];  // grid: edge and corner weight of ring j
+
+    LpOp(int p, int r) : pattern(p), R(r), pairs(make_pairs(p, r))
+    {
+        for (int j = 1; j <= R; ++j) {
+            ce[j] = (float)weight_ab(PRKO_GRID, R, 0, j);  // taps (+-j, |dj| < j) and (|di| < j, +-j)
+            cc[j] = (float)weight_ab(PRKO_GRID, R, j, j);  // taps (j, j) and (-j, -j)
+        }
+    }
+
+    template <typename X>
+    float point(const X &x) const
+    {
+        float acc = 0.0f;
+        if (pattern == PRKO_STAR) {
+            for (size_t k = 0; k < pairs.size(); ++k) {
+                const float diff = x(pairs[k].di, pairs[k].dj) - x(-pairs[k].di, -pairs[k].dj);
+                acc = (k == 0) ? pairs[k].w * diff : std::fmaf(pairs[k].w, diff, acc);
+            }
+            return acc;
+        }
+        // Grid: ring j = max(|di|, |dj|) has one weight on its edges,
+        // +ce[j] on column +j and row +j, -ce[j] on column -j and row -j
+        // (the corners excluded), and +-cc[j] on the corners (j, j) and
+        // (-j, -j).  So S = sum_j ce[j] * (V_j + H_j) + cc[j] * C_j with
+        //   V_j = VS_{j-1}(+j) - VS_{j-1}(-j), VS_k(b) the sum of column b
+        //         over rows -k..k: VS_0(b) = x(0, b),
+        //         VS_k(b) = VS_{k-1}(b) + (x(k, b) + x(-k, b));
+        //   H_j = D_j(0), then + D_j(-m), + D_j(+m) for m = 1..j-1, with
+        //         D_j(b) = x(j, b) - x(-j, b);
+        //   C_j = x(j, j) - x(-j, -j);
+        // acc = ce[1] * (H_1 + V_1), then fmaf(cc[j], C_j, acc) and
+        // fmaf(ce[j+1], H_{j+1} + V_{j+1}, acc) in ring order.
+        float vs[2 * max_radius + 1];
+        for (int b = -R; b <= R; ++b) vs[b + R] = x(0, b);
+        for (int j = 1; j <= R; ++j) {
+            const float vj = vs[R + j] - vs[R - j];
+            float hj = x(j, 0) - x(-j, 0);
+            for (int m = 1; m < j; ++m) {
+                hj = hj + (x(j, -m) - x(-j, -m));
+                hj = hj + (x(j, m) - x(-j, m));
+            }
+            const float tj = hj + vj;
+            const float cj = x(j, j) - x(-j, -j);
+            acc = (j == 1) ? ce[j] * tj : std::fmaf(ce[j], tj, acc);
+            acc = std::fmaf(cc[j], cj, acc);
+            if (j < R)
+                for (int b = -R; b <= R; ++b) vs[b + R] = vs[b + R] + (x(j, b) + x(-j, b));
+        }
+        return acc;
+    }
+};
+
 template <typename T>
-void lp_sweep(const std::vector<Pair> &pairs, int R, int n, int row_lo, int row_hi, long row0,
-              const T *in, T *out)
+void lp_sweep(const LpOp &op, int R, int n, int row_lo, int row_hi, long row0, const T *in, T *out)
 {
     // rows [row_lo,row_hi) of the global grid; arrays start at global row row0
     for (int i = std::max(row_lo, R); i < std::min(row_hi, n - R); ++i)
         for (int j = R; j < n - R; ++j) {
             const long p = (long)(i - row0) * n + j;
-            float acc = 0.0f;
-            for (size_t k = 0; k < pairs.size(); ++k) {
-                const long d = (long)pairs[k].di * n + pairs[k].dj;
-                const float diff = ld(in, p + d) - ld(in, p - d);
-                acc = (k == 0) ? pairs[k].w * diff : std::fmaf(pairs[k].w, diff, acc);
-            }
+            const float acc = op.point([&](int a, int b) { return ld(in, p + (long)a * n + b); });
             out[p] = st<T>(ld(out, p) + acc);
         }
 }
@@ -244,9 +297,9 @@ int lp_band(int pattern, int R, int n, int K, int init, uint64_t seed, int a, in
     std::vector<T> in((size_t)rows * n), out((size_t)rows * n);
     for (int i = lo; i < hi; ++i)
         lp_init_row<T>(init, seed, n, i, &in[(size_t)(i - lo) * n], &out[(size_t)(i - lo) * n]);
-    const std::vector<Pair> pairs = make_pairs(pattern, R);
+    const LpOp op(pattern, R);
     for (int it = 0; it <= K; ++it) {
-        lp_sweep<T>(pairs, R, n, a, b, lo, in.data(), out.data());
+        lp_sweep<T>(op, R, n, a, b, lo, in.data(), out.data());
         for (auto &x : in) x = st<T>(ld(&x, 0) + 1.0f);
     }
     std::memcpy(out_band, &out[(size_t)(a - lo) * n], sizeof(T) * (size_t)(b - a) * n);
@@ -407,6 +460,21 @@ int prko_band_lp(int dtype, int pattern, int radius, int n, int iterations, int 
     if (dtype == PRKO_BF16)
         return lp_band<uint16_t>(pattern, radius, n, iterations, init, seed, a, b,
                                  (uint16_t *)out_band, nullptr);
+    return -1;
+}
+
+int prko_sweep_lp(int dtype, int pattern, int radius, int n, const void *in, void *out)
+{
+    if (!valid_pr(pattern, radius) || n < 1 || 2 * radius + 1 > n) return -1;
+    const LpOp op(pattern, radius);
+    if (dtype == PRKO_F32) {
+        lp_sweep<float>(op, radius, n, 0, n, 0, (const float *)in, (float *)out);
+        return 0;
+    }
+    if (dtype == PRKO_BF16) {
+        lp_sweep<uint16_t>(op, radius, n, 0, n, 0, (const uint16_t *)in, (uint16_t *)out);
+        return 0;
+    }
     return -1;
 }
 

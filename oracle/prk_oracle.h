@@ -78,19 +78,36 @@ int prko_band(int pattern, int radius, int n, int iterations, int init,
               uint64_t seed, int a, int b, double *out_band);
 
 /* The reduced-precision storage forms, storage dtype PRKO_F32 or PRKO_BF16
- * (raw uint16 bit patterns), FP32 accumulation.  Arithmetic per point:
- *   acc = w1*(x[p+d1]-x[p-d1]); acc = fmaf(wk, x[p+dk]-x[p-dk], acc) ...
- * over the taps of positive weight (every stencil is antisymmetric,
- * W(-d) = -W(d)), ordered by row-offset magnitude |di| = 0..R and in
- * generator order within one magnitude, then out = T(float(out) + acc) and
- * in = T(float(in) + 1).  BF16 rounding is round-to-nearest-even.
- * Initial values: T(float(i+j)) or T(float(seeded double)). */
+ * (raw uint16 bit patterns), FP32 accumulation, weights rounded to float.
+ * Every stencil is antisymmetric (W(-d) = -W(d)).  Arithmetic per point,
+ * x(a, b) the input at row offset a, column offset b:
+ *   star: acc = w1*(x(d1)-x(-d1)); acc = fmaf(wk, x(dk)-x(-dk), acc) ...
+ *     over the taps of positive weight, ordered by row-offset magnitude
+ *     |di| = 0..R and in generator order within one magnitude;
+ *   grid: ring sums.  Ring j carries +-ce_j = +-1/(4j(2j-1)R) on its edges
+ *     (columns +-j and rows +-j, corners excluded) and +-cc_j = +-1/(4jR) on
+ *     the corners (j, j), (-j, -j):
+ *       VS_0(b) = x(0, b), VS_j(b) = VS_{j-1}(b) + (x(j, b) + x(-j, b))
+ *       V_j = VS_{j-1}(j) - VS_{j-1}(-j)
+ *       H_j = D_j(0) + D_j(-1) + D_j(1) + ... + D_j(-(j-1)) + D_j(j-1),
+ *             left to right, D_j(b) = x(j, b) - x(-j, b)
+ *       C_j = x(j, j) - x(-j, -j)
+ *       acc = ce_1*(H_1 + V_1); acc = fmaf(cc_1, C_1, acc);
+ *       acc = fmaf(ce_j, H_j + V_j, acc); acc = fmaf(cc_j, C_j, acc) ...
+ *     for j = 2..R;
+ * then out = T(float(out) + acc) and in = T(float(in) + 1).  BF16 rounding
+ * is round-to-nearest-even.  Initial values: T(float(i+j)) or
+ * T(float(seeded double)). */
 int prko_run_lp(int dtype, int pattern, int radius, int n, int iterations,
                 int init, uint64_t seed, void *out, void *in_final);
 
 /* Row band [a,b) of prko_run_lp (out_band: (b-a)*n elements of dtype). */
 int prko_band_lp(int dtype, int pattern, int radius, int n, int iterations,
                  int init, uint64_t seed, int a, int b, void *out_band);
+
+/* One reduced-precision sweep out += S(in) over the interior of given n x n
+ * arrays of dtype (no in += 1). */
+int prko_sweep_lp(int dtype, int pattern, int radius, int n, const void *in, void *out);
 
 /* L1 norm of a reduced-precision out grid, values widened to double. */
 double prko_l1norm_lp(int dtype, int n, int radius, const void *out);

@@ -47,6 +47,7 @@ class Oracle:
         L.prko_band.argtypes = [ctypes.c_int] * 5 + [_u64, ctypes.c_int, ctypes.c_int, _dp]
         L.prko_run_lp.argtypes = [ctypes.c_int] * 6 + [_u64, _vp, _vp]
         L.prko_band_lp.argtypes = [ctypes.c_int] * 6 + [_u64, ctypes.c_int, ctypes.c_int, _vp]
+        L.prko_sweep_lp.argtypes = [ctypes.c_int] * 4 + [_vp, _vp]
         L.prko_l1norm_lp.restype = ctypes.c_double
         L.prko_l1norm_lp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.prko_f32_to_bf16.restype = ctypes.c_uint16
@@ -110,6 +111,15 @@ class Oracle:
         assert self.L.prko_band_lp(dtype, pattern, radius, n, iterations, init, seed, a, b,
                                    out.ctypes.data) == 0
         return out
+
+    def sweep_lp(self, dtype, pattern, radius, inp, out) -> None:
+        """One reduced-precision sweep out += S(inp) in place (n x n arrays of
+        the storage dtype)."""
+        n = inp.shape[0]
+        assert inp.dtype == NP_LP[dtype] and out.dtype == NP_LP[dtype]
+        assert inp.shape == (n, n) and out.shape == (n, n)
+        assert inp.flags.c_contiguous and out.flags.c_contiguous
+        assert self.L.prko_sweep_lp(dtype, pattern, radius, n, inp.ctypes.data, out.ctypes.data) == 0
 
     def l1norm_lp(self, dtype, n, radius, out) -> float:
         return self.L.prko_l1norm_lp(dtype, n, radius, np.ascontiguousarray(out).ctypes.data)

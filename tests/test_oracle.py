@@ -212,6 +212,48 @@ def test_lp_forms_close_to_f64(oracle):
     assert np.array_equal(bf16_bits_to_f64(b16in), i[:, None] + i[None, :] + K + 1)
 
 
+@pytest.mark.parametrize("pattern,radius", OPS_ALL)
+def test_lp_operator_on_exact_inputs(oracle, pattern, radius):
+    """One FP32 sweep against the FP64 operator on integer inputs.
+
+    Every sum and difference the FP32 form builds from integers in
+    [-1000, 1000] stays below 2^24 in magnitude, so it is exact: the only
+    roundings are the weights' to float and the multiply / fma steps (2R for
+    a grid, one per tap pair for a star).  So at every point the result is
+    within (steps + 1) * 2^-24 * sum_k |w_k x_k| of the FP64 operator.  A
+    dropped, doubled or misplaced tap, or a wrong sign, moves a point by
+    |w_k x_k|, at least 2.6e-4 * |x_k|: far beyond the bound at almost every
+    point.  The BF16 form on BF16-exact integers is the FP32 result rounded
+    to BF16, point for point."""
+    n = 2 * radius + 1 + 24
+    rng = np.random.default_rng(700 + 10 * pattern + radius)
+    x = rng.integers(-1000, 1001, (n, n)).astype(np.float64)
+    ref = np.zeros((n, n))
+    oracle.stencil(pattern, radius, n, n, x, ref)
+    out = np.zeros((n, n), np.float32)
+    oracle.sweep_lp(F32, pattern, radius, x.astype(np.float32), out)
+    inner = (slice(radius, n - radius),) * 2
+    A = np.zeros((n, n))
+    for di, dj, w in oracle.taps(pattern, radius):
+        A[inner] += abs(w) * np.abs(np.roll(x, (-di, -dj), axis=(0, 1)))[inner]
+    steps = 2 * radius  # a grid's rings, or a star's tap pairs
+    err = np.abs(out.astype(np.float64) - ref)
+    assert np.all(err[inner] <= (steps + 1) * 2.0 ** -24 * A[inner])
+    assert np.count_nonzero(A[inner] > 0) == (n - 2 * radius) ** 2
+    frame = np.ones((n, n), bool)
+    frame[inner] = False
+    assert np.all(out[frame] == 0.0)
+    # BF16: integers in [-128, 128] are exact with an 8-bit significand
+    xb = rng.integers(-128, 129, (n, n)).astype(np.float32)
+    f = np.zeros((n, n), np.float32)
+    oracle.sweep_lp(F32, pattern, radius, xb, f)
+    bits = np.array([oracle.f32_to_bf16(float(v)) for v in xb.ravel()], np.uint16).reshape(n, n)
+    ob = np.zeros((n, n), np.uint16)
+    oracle.sweep_lp(BF16, pattern, radius, bits, ob)
+    expect = np.array([oracle.f32_to_bf16(float(v)) for v in f.ravel()], np.uint16).reshape(n, n)
+    assert np.array_equal(ob, expect)
+
+
 def test_lp_band_equals_full(oracle):
     n, K = 50, 3
     full, _ = oracle.run_lp(F32, 1, 3, n, K, 1, 8)
