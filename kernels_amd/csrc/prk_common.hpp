@@ -84,7 +84,7 @@ __host__ __device__ constexpr TapList make_taps(int pattern, int R)
 
 // Taps of positive weight.  Every stencil here is antisymmetric
 // (W(-d) = -W(d)), so S(x)[p] = sum_k w_k (x[p+d_k] - x[p-d_k]) over these
-// taps; the reduced-precision forms evaluate that paired form.  Order: by
+// taps; the reduced-precision star forms evaluate that paired form.  Order: by
 // row-offset magnitude |di| = 0..R, generator order within one magnitude, so
 // a kernel needs only the two rows p±|di| at a time.
 __host__ __device__ constexpr TapList make_pairs(int pattern, int R)

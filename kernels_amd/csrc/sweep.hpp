@@ -36,8 +36,9 @@
 // Arithmetic.  FP64: the products are summed in the generator's order and
 // the sum is added to out last, with contraction off, so results are
 // bitwise those of the reference's serial operator.  FP32 / BF16 storage:
-// FP32 accumulation over antisymmetric pairs, acc = w1*(x[p+d1]-x[p-d1]),
-// then fma for the remaining pairs, out = T(float(out) + acc).
+// FP32 accumulation, out = T(float(out) + acc); for stars over antisymmetric
+// pairs, acc = w1*(x[p+d1]-x[p-d1]), then fma for the remaining pairs; for
+// grids from ring sums (ring_step).
 #pragma once
 
 #include "prk_common.hpp"
@@ -297,15 +298,15 @@ __device__ inline void pair_sums_f64(const double *const (&rows)[2 * R + 1], dou
 // around the lane's own V columns (window index i = column i - HP); a row
 // whose taps reach MAXD columns to either side needs the indices
 // [HP-MAXD, HP+V-1+MAXD].  Each 16-byte vector holding a needed column is
-// read from LDS and widened to float once; the paired sum then reads the
-// register window.  With WHOLE, an empty asm keeps every read whole: left to
-// itself the compiler narrows the partly used edge vectors to 2- and 4-byte
-// LDS reads at a 16-byte lane stride, which conflicted in the LDS banks for
-// 18.7 % of the BF16 grid R=3 sweep's cycles.  Measured on MI355X in
-// alternating runs, whole reads made BF16 grid R=5 16-18 % and R=7 8 %
-// faster, but BF16 grid R=2, 3, 4, 6 and 8 0-5 % slower (the asm also
-// constrains the scheduler), FP32 grid R=5 10 % slower and BF16 star R=5 no
-// faster, so only the BF16 grid R=5 and R=7 sums use them.
+// read from LDS and widened to float once; the sums then read the register
+// window.  With WHOLE, an empty asm keeps every read whole: left to itself
+// the compiler narrows the partly used edge vectors to 2- and 4-byte LDS
+// reads at a 16-byte lane stride, which conflict in the LDS banks (19.8 % of
+// the BF16 grid R=3 sweep's cycles).  Measured on MI355X with the grid ring
+// sums, alternating, at fixed tilings: whole reads make BF16 grid R=5 37 %
+// and R=7 21 % faster, but BF16 grid R=2, 3, 4 and 6 0-3.5 % and R=8 10 %
+// slower (the asm also constrains the scheduler), so only the BF16 grid R=5
+// and R=7 sums use them.
 template <typename T, int V, int HP, int MAXD, bool WHOLE>
 __device__ inline void load_window(const T *row, float (&w)[V + 2 * HP])
 {
@@ -334,8 +335,9 @@ constexpr int pair_max_dj(int a)
     return m;
 }
 
-// acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs, in the pair order
-// of make_pairs (by row-offset magnitude |di|, generator order within one).
+// Stars: acc = w1*(x[+d1]-x[-d1]), then fma over the other pairs, in the
+// pair order of make_pairs (by row-offset magnitude |di|, generator order
+// within one).
 // The pairs of one magnitude a read only the rows +a and -a, so the sum walks
 // the magnitudes 0..R with two widened rows live at a time.
 template <int P, int R>
@@ -378,14 +380,13 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
     constexpr int b = pair_begin<P, R>(A), e = pair_begin<P, R>(A + 1);
     if constexpr (e > b) {
         constexpr int md = pair_max_dj<P, R>(A);
-        constexpr bool whole = P == GRID && sizeof(T) == 2 && (R == 5 || R == 7);
         float wp[V + 2 * HP];
-        load_window<T, V, HP, md, whole>(rows[R + A], wp);
+        load_window<T, V, HP, md, false>(rows[R + A], wp);
         if constexpr (A == 0) {
             pair_range<P, R, V, HP, b>(wp, wp, acc, std::make_index_sequence<e - b>{});
         } else {
             float wm[V + 2 * HP];
-            load_window<T, V, HP, md, whole>(rows[R - A], wm);
+            load_window<T, V, HP, md, false>(rows[R - A], wm);
             pair_range<P, R, V, HP, b>(wp, wm, acc, std::make_index_sequence<e - b>{});
         }
     }
@@ -407,11 +408,12 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
 //   acc = ce_1 * (H_1 + V_1), then fmaf(cc_j, C_j, acc) and
 //   fmaf(ce_{j+1}, H_{j+1} + V_{j+1}, acc) in ring order.
 // The column sums VS and the differences D are formed once per column and
-// shared by the lane's V points.  Per point at R=3 that is 29 FP32
-// operations (BF16, V=8) or 32 (FP32, V=4) against the 42 of the paired
-// form, whose every tap pair costs a subtraction and an fma: the BF16 grid
-// sweeps issue VALU instructions near the CUs' rate, the HBM-bound ones
-// do not notice.
+// shared by the lane's V points.  Per point at R=3 that is 32 FP32
+// operations (BF16, V=8) or 35 (FP32, V=4) against the 42 of the paired
+// form (a subtraction and an fma per tap pair), at R=8 153 or 188 against
+// 272.  The BF16 grid sweeps and the large-R grids are bound by their VALU
+// issue and run faster for it (BF16 grid R=3 +2-3 %, R=5 +25 %, R=8 +46 %;
+// FP32 grid R=5 +14 %, R=8 +28 %); the HBM-bound ones do not notice.
 // ---------------------------------------------------------------------------
 template <int R, int J>
 constexpr float ring_edge_w() { return (float)weight_ab(GRID, R, 0, J); }   // tap (J, 0)
@@ -1192,7 +1194,7 @@ struct RingSlot {
     }
 };
 
-// Paired sums over the ring: the window of row o + a is in slot
+// Star paired sums over the ring: the window of row o + a is in slot
 // (k + 1 + R + a) % S for the output row of ring step k (rows o-R..o+R sit
 // in slots k+1..k+S).  k is the step of the fully unrolled row loop, so every
 // index is a constant once the loop is unrolled and the ring stays in

@@ -159,9 +159,9 @@ def test_batch_autotune_keeps_results(gpu):
 def fp32_normwise_bound(iters: int) -> float:
     """Normwise error of an FP32-storage solve against FP64 after iters+1
     sweeps grows with the sweeps (the FP32 rounding of in_0 + k enters each
-    one): over the config-4 sample it measured 1.6e-7 at K=4, 6.4e-7 to
-    7.1e-7 at K=28..33 and 1.12e-6 at K=50 (oracle arithmetic, which the
-    kernels match bit for bit).  The bound is 1.3x to 1.9x those."""
+    one): over the config-4 sample it measured 1.1e-7 at K=4, 6.2e-7 to
+    7.2e-7 at K=28..33 and 1.16e-6 at K=50 (oracle arithmetic, which the
+    kernels match bit for bit).  The bound is 1.6x to 1.9x those."""
     return 3.5e-8 * (iters + 2)
 
 

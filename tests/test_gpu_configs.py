@@ -132,9 +132,9 @@ def errors(x: torch.Tensor, y: torch.Tensor, radius: int) -> dict:
 # about 1.5x the largest value measured over the eight stencils.  FP32:
 # analytic init keeps every value an integer below 2^24 or a sum of rounded
 # weights, so the error is a few FP32 rounding steps (at most 7.5e-8,
-# star R=5 and grid R=3); seeded init stores in_0 + k in FP32, whose
-# rounding (ulp 2^-18 near 51) enters every sweep's stencil: 1.12e-6 to
-# 1.13e-6 in every case, above the 1e-6 the FP32 form aims for.  BF16
+# star R=5); seeded init stores in_0 + k in FP32, whose rounding (ulp 2^-18
+# near 51) enters every sweep's stencil: 1.12e-6 to 1.15e-6 in every case,
+# above the 1e-6 the FP32 form aims for.  BF16
 # (8-bit significand) stores in_0 + k to within 2^-3 near 51 and i + j + k
 # to within 64 near 16384: its error against FP64 is of the order of the
 # result itself with analytic init (normwise 3.6 to 6.7) and 0.073 to 0.075
