@@ -261,6 +261,16 @@ int prk_batch_run(prk_batch *b, void *stream);
 int prk_batch_wait(prk_batch *b, double *norms);
 int prk_batch_output(const prk_batch *b, int idx, int which, void *host, size_t bytes);
 int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles, int *launches);
+/* Error against FP64 of every result of a batch (no reference counterpart;
+ * the reference validates each run's norm, Cxx11/stencil.cc:209-216).
+ * `ref` holds the same solves (pattern, radius, n, iterations, init, seed,
+ * in the same order) stored in FP64; both batches have run and been waited
+ * for.  For solve i, err[3i], err[3i+1], err[3i+2] receive max |x - y|,
+ * mean |x - y| and ||x - y||_2 / ||y||_2 over the interior, x the solve's
+ * out in `b` widened to double and y its out in `ref` (zeros when the solve
+ * is stored in FP64 in `b`, or when ||y|| is 0 for the last).  Computed on
+ * the device in a fixed order (the same results on every call); blocks. */
+int prk_batch_errors(const prk_batch *b, const prk_batch *ref, double *err);
 void prk_batch_destroy(prk_batch *b);
 /* Tile heights for batched solves, tuned per (pattern, radius, dtype, size
  * class floor(log2 n)).  prk_batch_autotune groups `solves` by that key,

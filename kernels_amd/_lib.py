@@ -83,6 +83,7 @@ SIGNATURES.update({
     "prk_batch_wait": (c_int, [c_void_p, c_double_p]),
     "prk_batch_output": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_size_t]),
     "prk_batch_info": (c_int, [c_void_p, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll), c_int_p]),
+    "prk_batch_errors": (c_int, [c_void_p, c_void_p, c_double_p]),
     "prk_batch_destroy": (None, [c_void_p]),
     "prk_solve_one_by_one": (c_int, [_SOLVE_P, c_int, c_int, c_double_p, c_void_p]),
     "prk_set_batch_round_tiles": (c_int, [c_ll]),

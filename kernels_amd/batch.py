@@ -54,6 +54,15 @@ class Batch:
         check(lib().prk_batch_info(self.h, ctypes.byref(b), ctypes.byref(t), ctypes.byref(q)))
         return {"device_bytes": b.value, "tiles": t.value, "launches": q.value}
 
+    def errors(self, ref: "Batch") -> np.ndarray:
+        """Error of every result against `ref`, the same solves stored in
+        FP64 (both run and waited for): rows (max abs, mean abs, normwise
+        L2) per solve, zeros for solves stored in FP64 here
+        (prk_batch_errors)."""
+        e = np.zeros((self.count, 3))
+        check(lib().prk_batch_errors(self.h, ref.h, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return e
+
     def close(self) -> None:
         if self.h:
             lib().prk_batch_destroy(self.h)

@@ -458,6 +458,50 @@ int prk_batch_info(const prk_batch *b, long long *device_bytes, long long *tiles
     return PRK_OK;
 }
 
+int prk_batch_errors(const prk_batch *b, const prk_batch *ref, double *err)
+{
+    if (!b || !ref || !err) return set_error(PRK_EINVAL, "bad batch errors request");
+    const size_t count = b->solves.size();
+    if (ref->solves.size() != count) return set_error(PRK_EINVAL, "the batches hold different solve counts");
+    for (size_t i = 0; i < count; ++i) {
+        const prk_solve &x = b->solves[i], &y = ref->solves[i];
+        if (y.dtype != PRK_F64 || x.pattern != y.pattern || x.radius != y.radius || x.n != y.n ||
+            x.iterations != y.iterations || x.init != y.init || x.seed != y.seed)
+            return set_error(PRK_EINVAL, "solve %zu of the reference batch is not the same solve in FP64", i);
+    }
+    // partials (kErrBlocks x 4) then 4 results per solve
+    double *d = nullptr;
+    const size_t doubles = (size_t)kErrBlocks * 4 + 4 * count;
+    if (int rc = check_hip(hipMalloc((void **)&d, doubles * sizeof(double)), "batch errors allocation")) return rc;
+    std::vector<double> h(4 * count, 0.0);
+    hipStream_t s = nullptr;
+    int rc = check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "batch errors stream");
+    // the batches ran on their callers' streams: let that work finish first
+    if (!rc) rc = check_hip(hipDeviceSynchronize(), "batch errors wait");
+    for (size_t i = 0; i < count && !rc; ++i) {
+        if (b->solves[i].dtype == PRK_F64) continue;
+        const BGrid &gx = b->grids[i], &gy = ref->grids[i];
+        rc = error_vs_f64_launch(b->solves[i].dtype, gx.out, gx.pitch, (const double *)gy.out, gy.pitch, gx.n,
+                                 gx.radius, d, d + (size_t)kErrBlocks * 4 + 4 * i, s);
+    }
+    if (!rc)
+        rc = check_hip(hipMemcpyAsync(h.data(), d + (size_t)kErrBlocks * 4, 4 * count * sizeof(double),
+                                      hipMemcpyDeviceToHost, s),
+                       "batch errors copy");
+    if (!rc) rc = check_hip(hipStreamSynchronize(s), "batch errors wait");
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(d);
+    if (rc) return rc;
+    for (size_t i = 0; i < count; ++i) {
+        const double *v = &h[4 * i];
+        const double m = (double)(b->solves[i].n - 2 * b->solves[i].radius);
+        err[3 * i] = b->solves[i].dtype == PRK_F64 ? 0.0 : v[0];
+        err[3 * i + 1] = b->solves[i].dtype == PRK_F64 ? 0.0 : v[1] / (m * m);
+        err[3 * i + 2] = (b->solves[i].dtype == PRK_F64 || v[3] == 0.0) ? 0.0 : std::sqrt(v[2] / v[3]);
+    }
+    return PRK_OK;
+}
+
 void prk_batch_destroy(prk_batch *b) { batch_free(b); }
 
 int prk_batch_autotune(const prk_solve *solves, int count, int sample, int reps, int *keys_tuned)

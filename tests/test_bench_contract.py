@@ -59,6 +59,10 @@ def test_default_workload_line(gpu):
     bat = d["batched_config4"]
     assert bat["solves_per_s"] > bat["one_by_one_f64_solves_per_s"] > 0
     assert bat["max_rel_norm_diff_vs_f64"] < 1e-5
+    # every batched FP32 result has its error against FP64, within the bound
+    errs = bat["fp32_vs_fp64_all"]
+    assert errs["solves"] == errs["fp32_solves"] == 4096
+    assert errs["fp32_within_bound"] == 4096 and 0 < errs["normwise_l2"] < 2e-6
 
 
 @pytest.mark.gpu

@@ -56,6 +56,41 @@ def test_batch_matches_oracle(gpu, oracle, flags):
     b.close()
 
 
+def test_batch_errors_against_f64(gpu, oracle):
+    # prk_batch_errors over a mix of storage types: FP64 solves get zeros,
+    # FP32 and BF16 solves the errors of their oracle results against the
+    # FP64 oracle (the max exactly, the sums to 1e-12); two calls agree bit
+    # for bit; a reference batch that is not the same solves in FP64 is
+    # refused.
+    b = Batch(MIX)
+    b.run()
+    b.wait()
+    ref = Batch([(s[0], s[1], F64) + tuple(s[3:]) for s in MIX])
+    ref.run()
+    ref.wait()
+    errs = b.errors(ref)
+    assert np.array_equal(errs.view(np.uint64), b.errors(ref).view(np.uint64))
+    for i, s in enumerate(MIX):
+        pattern, radius, dtype, n, iters, init, seed = s
+        print(f"solve {i}: {s} errors {errs[i]}", flush=True)
+        if dtype == F64:
+            assert np.all(errs[i] == 0.0)
+            continue
+        x = ref_solve(oracle, s)[0]
+        x = (x.astype(np.uint32) << 16).view(np.float32) if dtype == BF16 else x
+        y, _ = oracle.run(pattern, radius, n, 32, iters, init, seed)
+        inner = (slice(radius, n - radius),) * 2
+        d = np.abs(x.astype(np.float64)[inner] - y[inner])
+        normwise = np.linalg.norm(d) / np.linalg.norm(y[inner])
+        assert errs[i, 0] == d.max(), (s, errs[i, 0], d.max())
+        assert abs(errs[i, 1] - d.mean()) <= 1e-12 * d.mean(), (s, errs[i, 1], d.mean())
+        assert abs(errs[i, 2] - normwise) <= 1e-12 * normwise, (s, errs[i, 2], normwise)
+    with pytest.raises(RuntimeError):
+        b.errors(b)  # not stored in FP64
+    ref.close()
+    b.close()
+
+
 @pytest.mark.parametrize("flags", FORMS)
 def test_alone_equals_in_mix_and_replays(gpu, flags):
     mix = MIX * 3
@@ -171,8 +206,10 @@ def test_config4_mix(gpu, oracle):
     # every batched norm equals the one-grid-per-launch FP32 norm bit for
     # bit; a spread sample (every 400th solve, the largest grid and the
     # largest grid R=4 at K=50) equals the oracle bit for bit on three row
-    # bands (the whole grid up to n=600), in FP32 and in FP64; and each
-    # sampled FP32 result is within the stated bound of its FP64 result.
+    # bands (the whole grid up to n=600), in FP32 and in FP64; every FP32
+    # result is within the stated bound of its FP64 result by the device's
+    # per-solve errors (prk_batch_errors), whose figures equal the host's on
+    # the sample.
     import bench
     from concurrent.futures import ThreadPoolExecutor
 
@@ -186,17 +223,21 @@ def test_config4_mix(gpu, oracle):
     b.run()
     norms = b.wait()
     outs = {i: (b.output(i, 0), b.output(i, 1)) for i in sample}
+    print("config 4: the same 4096 solves in FP64", flush=True)
+    b64 = Batch([(s[0], s[1], F64) + tuple(s[3:]) for s in specs])
+    b64.run()
+    b64.wait()
+    outs64 = [(b64.output(i, 0), b64.output(i, 1)) for i in sample]
+    errs = b.errors(b64)
+    b64.close()
     b.close()
+    bounds = np.array([fp32_normwise_bound(s[4]) for s in specs])
+    print(f"config 4: FP32 normwise error against FP64 over 4096 solves: max {errs[:, 2].max():.3e}, "
+          f"median {np.median(errs[:, 2]):.3e}, max of error / bound {np.max(errs[:, 2] / bounds):.3f}", flush=True)
+    assert errs[:, 2].max() > 0 and np.all(errs[:, 2] <= bounds), int(np.sum(errs[:, 2] > bounds))
     print("config 4: one grid per launch", flush=True)
     n32 = one_by_one(specs)
     assert np.array_equal(norms, n32), int(np.sum(norms != n32))
-    print("config 4: FP64 sample", flush=True)
-    s64 = [(specs[i][0], specs[i][1], F64) + tuple(specs[i][3:]) for i in sample]
-    b64 = Batch(s64)
-    b64.run()
-    b64.wait()
-    outs64 = [(b64.output(j, 0), b64.output(j, 1)) for j in range(len(sample))]
-    b64.close()
 
     def spans(s):
         n, r = s[3], s[1]
@@ -228,3 +269,8 @@ def test_config4_mix(gpu, oracle):
         normwise = float(np.linalg.norm(dd) / np.linalg.norm(y))
         print(f"config 4 solve {i} {specs[i]}: FP32 normwise {normwise:.3e} max abs {dd.max():.3e}", flush=True)
         assert normwise <= fp32_normwise_bound(k), (specs[i], normwise)
+        # the device's figures for the same solve: the max exactly, the sums
+        # (summed in another order) to 1e-12
+        assert errs[i, 0] == dd.max(), (specs[i], errs[i, 0], dd.max())
+        assert abs(errs[i, 1] - dd.mean()) <= 1e-12 * dd.mean(), (specs[i], errs[i, 1], dd.mean())
+        assert abs(errs[i, 2] - normwise) <= 1e-12 * normwise, (specs[i], errs[i, 2], normwise)
