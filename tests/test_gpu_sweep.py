@@ -225,6 +225,33 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
         K.check(L.prk_set_sweep_order(-1, -1)); K.check(L.prk_tuning_clear())
 
 
+@pytest.mark.parametrize("dtype", [F32, BF16])
+@pytest.mark.parametrize("radius", [2, 3, 5])
+def test_grid_ring_sums_every_tiling(gpu, oracle, dtype, radius):
+    # The FP32/BF16 grid ring sums in every pipeline variant of the LDS-ring
+    # sweep (1-3 rows in flight, cached or non-temporal stores) and, at
+    # R <= 3, the register window, over several row heights, tile orders and
+    # occupancies: bitwise the oracle's every time (BF16 grid R=5 reads its
+    # windows whole, the others narrow).
+    n, iters, pattern = 301, 3, 1
+    L = K.lib()
+    ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 21)
+    tilings = [(0, 1, 0, 0), (1, 5, 2, 1), (2, 64, 1, 4), (3, 7, 3, 3), (4, 8, 1, 2), (5, 12, 1, 4), (5, 300, 0, 1)]
+    if radius <= 3:
+        tilings += [(6, 6, 1, 4), (7, 13, 1, 2)]
+    try:
+        for v, rows, order, wgs in tilings:
+            K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
+            K.check(L.prk_set_sweep_order(order, wgs))
+            b0, b1, out = (K.alloc_grid(n, dtype) for _ in range(3))
+            K.init(K.INIT_SEEDED, 21, dtype, b0, out)
+            K.run(pattern, radius, dtype, iters, b0, b1, out)
+            assert np.array_equal(bits(to_host(out)), bits(ref)), (v, rows, order, wgs)
+    finally:
+        K.check(L.prk_set_sweep_variant(-1)); K.check(L.prk_set_rows_per_wave(0))
+        K.check(L.prk_set_sweep_order(-1, -1)); K.check(L.prk_tuning_clear())
+
+
 EXT = [(p, r) for p in (0, 1) for r in (6, 7, 8)]
 
 
