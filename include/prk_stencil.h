@@ -418,7 +418,8 @@ int prk_transpose_check(int order, int iterations, const double *B, double *abse
 /* Tuning knob (process-wide, results unchanged): tile shape (rows x
  * columns of A, threads) 0 = 64x64, 1 = 128x64, 2 = 64x128 (default),
  * 3 = 32x64, 4 = 128x128 on 1024 threads, 5 = 32x256, 6 = 256x32,
- * 7 = 64x128 on 512 threads; tile order 0 = row-major (default), 1 =
+ * 7 = 64x128 on 512 threads, 8 = 128x128 on 1024 threads in two halves
+ * of 64 KiB of LDS; tile order 0 = row-major (default), 1 =
  * column-major, 2 = groups of eight tile rows; -1 = default.  Odd orders
  * always use 64x64 tiles with 8-byte accesses. */
 int prk_set_transpose_variant(int shape, int mode);

@@ -141,6 +141,85 @@ __global__ __launch_bounds__(NTH) void transpose_kernel(TransposeArgs a)
     }
 }
 
+// One 128 x 128 tile of A on 1024 threads, in two halves through 66 KiB of
+// LDS, so that both matrices move in 1 KiB row segments (the 64 x 128 tile
+// writes B in 512-byte segments) and two workgroups fit on a CU.  Each
+// thread loads 16 doubles of A (eight 16-byte loads, rows p*16 + t/64,
+// columns 2(t%64), 2(t%64)+1 of the tile) and writes A + 1 back at once.
+// Half h = 0, 1: the threads holding tile columns [64h, 64h+64) park them in
+// LDS (128 rows x 64 columns, rows padded by one element); after a barrier
+// every thread adds four 16-byte pieces of B rows 64h + q*16 + t/64 (tile
+// rows 2(t%64), 2(t%64)+1 as B columns) and stores them; a barrier ends the
+// half before the LDS is reused.
+template <int NT>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void transpose_kernel_2h(TransposeArgs a)
+{
+    constexpr int T = 128, H = 64, NTH = 1024, RPP = NTH / 64;  // 16 A rows per pass
+    constexpr int AUX_ST = NT ? 2 : 0;
+    __shared__ double half[T][H + 1];
+
+    int tr, tc;
+    tile_of_transpose(a, blockIdx.x, tr, tc);
+    const int order = a.order;
+    const unsigned row_bytes = (unsigned)order * 8u;
+    union V { u32x4 v; double d[2]; };
+    // lr: the wave's index, uniform (row resources stay in scalar registers)
+    const int lr = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l2 = (threadIdx.x & 63) * 2;
+
+    // A: tile rows p*16 + lr, columns l2, l2+1
+    V x[T / RPP];
+    {
+        const int c = tc * T + l2;
+        const unsigned off = c < order ? (unsigned)c * 8u : kOOB;
+        __amdgpu_buffer_rsrc_t rs[T / RPP];
+#pragma unroll
+        for (int p = 0; p < T / RPP; ++p) {
+            const int r = tr * T + p * RPP + lr;
+            const bool ok = r < order;
+            rs[p] = row_rsrc(a.A + (long long)(ok ? r : 0) * order, ok ? row_bytes : 0u);
+            x[p].v = bload16<0>(rs[p], off);
+        }
+#pragma unroll
+        for (int p = 0; p < T / RPP; ++p) {
+            V y;
+            y.d[0] = x[p].d[0] + 1.0;
+            y.d[1] = x[p].d[1] + 1.0;
+            bstore16<AUX_ST>(rs[p], off, y.v);
+        }
+    }
+    // B rows tc*T + 64h + q*16 + lr, columns tr*T + l2, l2+1
+    const int cb = tr * T + l2;
+    const unsigned offb = cb < order ? (unsigned)cb * 8u : kOOB;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if ((l2 >> 6) == h) {
+#pragma unroll
+            for (int p = 0; p < T / RPP; ++p) {
+                half[p * RPP + lr][l2 - H * h] = x[p].d[0];
+                half[p * RPP + lr][l2 - H * h + 1] = x[p].d[1];
+            }
+        }
+        __syncthreads();
+        V b[H / RPP];
+        __amdgpu_buffer_rsrc_t rs[H / RPP];
+#pragma unroll
+        for (int q = 0; q < H / RPP; ++q) {
+            const int y = tc * T + H * h + q * RPP + lr;
+            const bool ok = y < order;
+            rs[q] = row_rsrc(a.B + (long long)(ok ? y : 0) * order, ok ? row_bytes : 0u);
+            b[q].v = bload16<0>(rs[q], offb);
+        }
+#pragma unroll
+        for (int q = 0; q < H / RPP; ++q) {
+            const int j = q * RPP + lr;  // column of the half
+            b[q].d[0] = b[q].d[0] + half[l2][j];
+            b[q].d[1] = b[q].d[1] + half[l2 + 1][j];
+            bstore16<AUX_ST>(rs[q], offb, b[q].v);
+        }
+        __syncthreads();
+    }
+}
+
 // A = 0, 1, ..., order^2 - 1 (row-major), B = 0 (Cxx11/transpose.cc:110-112).
 __global__ __launch_bounds__(256) void transpose_init_kernel(double *A, double *B, long long nn)
 {
@@ -234,6 +313,15 @@ int launch_shape(TransposeArgs a, hipStream_t s)
     return check_hip(hipGetLastError(), "transpose launch");
 }
 
+int launch_2h(TransposeArgs a, hipStream_t s)
+{
+    a.ntr = (a.order + 127) / 128;
+    a.ntc = a.ntr;
+    const unsigned blocks = (unsigned)a.ntr * (unsigned)a.ntc;
+    hipLaunchKernelGGL(transpose_kernel_2h<1>, dim3(blocks), dim3(1024), 0, s, a);
+    return check_hip(hipGetLastError(), "transpose launch");
+}
+
 int launch_transpose(int order, double *A, double *B, hipStream_t s)
 {
     TransposeArgs a{A, B, order, 0, 0, g_tr_mode < 0 ? 0 : g_tr_mode};
@@ -249,6 +337,7 @@ int launch_transpose(int order, double *A, double *B, hipStream_t s)
     case 5: return launch_shape<32, 256, 2>(a, s);
     case 6: return launch_shape<256, 32, 2>(a, s);
     case 7: return launch_shape<64, 128, 2, 512>(a, s);
+    case 8: return launch_2h(a, s);
     default: return launch_shape<64, 64, 2>(a, s);
     }
 }
@@ -277,8 +366,8 @@ int prk_transpose_ex(int order, double *A, double *B, void *stream)
 
 int prk_set_transpose_variant(int shape, int mode)
 {
-    if (shape < -1 || shape > 7 || mode < -1 || mode > 2)
-        return set_error(PRK_EINVAL, "transpose shape must be -1..7 and tile order -1..2");
+    if (shape < -1 || shape > 8 || mode < -1 || mode > 2)
+        return set_error(PRK_EINVAL, "transpose shape must be -1..8 and tile order -1..2");
     g_tr_shape = shape;
     g_tr_mode = mode;
     return PRK_OK;

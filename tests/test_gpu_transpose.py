@@ -89,7 +89,7 @@ def test_transpose_tile_shapes_do_not_change_results(gpu, oracle):
     try:
         for order in (300, 301):
             rA, rB = oracle.transpose_run(order, 32, 2)
-            for shape in range(8):
+            for shape in range(9):
                 for mode in range(3):
                     K.check(L.prk_set_transpose_variant(shape, mode))
                     A = torch.empty((order, order), dtype=torch.float64, device=gpu)
@@ -99,4 +99,4 @@ def test_transpose_tile_shapes_do_not_change_results(gpu, oracle):
                     assert np.array_equal(host(A), rA) and np.array_equal(host(B), rB), (order, shape, mode)
     finally:
         K.check(L.prk_set_transpose_variant(-1, -1))
-    assert L.prk_set_transpose_variant(8, 0) == -1
+    assert L.prk_set_transpose_variant(9, 0) == -1
