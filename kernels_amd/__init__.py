@@ -12,7 +12,7 @@ from ._lib import (  # noqa: F401
     BF16, F32, F64, GRID, INIT_ANALYTIC, INIT_SEEDED, STAR, PrkError, Tuning, check, lib, lib_path,
 )
 from .ops import (  # noqa: F401
-    DTYPES, add, alloc_grid, alloc_slab, autotune, init, l1norm, nominal_flops, run, slab_bytes_per_iter, slab_halos, stencil,
+    DTYPES, add, alloc_grid, alloc_slab, autotune, default_pitch, init, l1norm, nominal_flops, run, slab_bytes_per_iter, slab_halos, stencil,
     stencil_bytes_per_iter, stencil_ex, sweep, sweep_slab, torch_dtype, OPERATORS, transpose,
     transpose_bytes_per_iter, transpose_check, transpose_init, transpose_run,
 )

@@ -293,7 +293,7 @@ int prk_batch_create(const prk_solve *solves, int count, int rows_per_tile, int 
         const int V = vec_of(s.dtype);
         BGrid &g = b->grids[i];
         std::memset(&g, 0, sizeof g);
-        g.pitch = (long long)align_up((size_t)s.n, (size_t)V);
+        g.pitch = lib_pitch(s.n, s.dtype);
         g.n = s.n;
         g.pattern = s.pattern;
         g.radius = s.radius;
@@ -652,8 +652,7 @@ int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override,
     for (auto &s : ss) {
         if (dtype_override >= 0) s.dtype = dtype_override;
         if ((rc = validate_solve(s))) return rc;
-        const size_t V = (size_t)vec_of(s.dtype);
-        need = std::max(need, (size_t)s.n * align_up((size_t)s.n, V) * dtype_size(s.dtype));
+        need = std::max(need, (size_t)s.n * (size_t)lib_pitch(s.n, s.dtype) * dtype_size(s.dtype));
     }
     need = align_up(need, 256);
     void *mem = nullptr;
@@ -671,7 +670,7 @@ int prk_solve_one_by_one(const prk_solve *solves, int count, int dtype_override,
     void *b0 = mem, *b1 = (char *)mem + need, *out = (char *)mem + 2 * need;
     for (int i = 0; i < count && rc == PRK_OK; ++i) {
         const prk_solve &s = ss[i];
-        const long long pitch = (long long)align_up((size_t)s.n, (size_t)vec_of(s.dtype));
+        const long long pitch = lib_pitch(s.n, s.dtype);
         rc = prk_init(s.init, s.seed, s.dtype, s.n, pitch, b0, out, st);
         SweepPlan p;
         if (!rc) rc = plan_sweep(s.pattern, s.radius, s.dtype, s.n, pitch, b0, b1, out, 0, &p);

@@ -171,7 +171,7 @@ size_t request_bytes(const prk_solve &s, BGrid &g)
 {
     const int V = vec_of(s.dtype);
     std::memset(&g, 0, sizeof g);
-    g.pitch = (long long)align_up((size_t)s.n, (size_t)V);
+    g.pitch = lib_pitch(s.n, s.dtype);
     g.n = s.n;
     g.pattern = s.pattern;
     g.radius = s.radius;
