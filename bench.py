@@ -935,6 +935,7 @@ def main(argv=None) -> int:
     # analytic closed form does not apply to it; its parity is checked
     # bitwise against the oracle's BF16 arithmetic in the tests instead.
     validated = None if dtype == 2 else abs(norm - 2.0 * sweeps) <= eps
+    row_pitch = out.stride(0)
     del bufs, out
     torch.cuda.empty_cache()
     if a.copy_calibration:
@@ -969,7 +970,7 @@ def main(argv=None) -> int:
         "config": {"workload": f"{'star' if pattern == 0 else 'grid'} R={radius} {DT_NAME[dtype]} "
                                f"n={n} fused sweep (out+=S(in), in_next=in+1)",
                    "pattern": "star" if pattern == 0 else "grid", "radius": radius, "n": n,
-                   "iterations": a.steps, "tile": 32},
+                   "iterations": a.steps, "tile": 32, "row_pitch_elements": row_pitch},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_tiling": " ".join(map(str, tiling_key)),

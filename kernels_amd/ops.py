@@ -94,11 +94,15 @@ def _check_dense(n: int, *grids) -> None:
 
 
 def default_pitch(n: int, dtype: int) -> int:
-    """Row stride (elements) of alloc_grid and alloc_slab: rows padded to 128
-    bytes, so that every row starts on a cache line (the vector sweeps need
-    16; rows on a line boundary ran 11-17 % faster on MI355X where n is not
-    a multiple of 128 bytes)."""
-    per = 128 // _ESIZE[dtype]
+    """Row stride (elements) of alloc_grid and alloc_slab.  Rows are padded to
+    128 bytes, so that every row starts on a cache line (the vector sweeps
+    need 16; on MI355X rows on a line boundary ran 11-17 % faster where n is
+    not a multiple of 128 bytes), and rows of 64 KiB or more to 1 KiB, so
+    that each wave's 1 KiB row segment starts on a 1 KiB boundary (at
+    n = 30000 that ran 1-2.6 % faster in FP64 and FP32 for at most 1.6 %
+    more memory)."""
+    es = _ESIZE[dtype]
+    per = (1024 if n * es >= 65536 else 128) // es
     return (n + per - 1) // per * per
 
 
