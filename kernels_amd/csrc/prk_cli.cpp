@@ -116,7 +116,11 @@ int main(int argc, char *argv[])
     // library refuses the operator and the run fails the same way.
     const int pattern = star ? PRK_STAR : PRK_GRID;
     const size_t es = dtype == PRK_F64 ? 8 : dtype == PRK_F32 ? 4 : 2;
-    const long long pitch = unfused ? n : ((long long)n * (long long)es + 15) / 16 * 16 / (long long)es;
+    // Fused runs pad rows to 128 bytes (a cache line), rows of 64 KiB or more
+    // to 1 KiB, as the Python helpers do; the unfused run keeps the
+    // reference's pitch n for its drop-in operators.
+    const long long row_align = ((long long)n * (long long)es >= 65536 ? 1024 : 128) / (long long)es;
+    const long long pitch = unfused ? n : ((long long)n + row_align - 1) / row_align * row_align;
     const size_t bytes = (size_t)n * (size_t)pitch * es;
 
     void *buf[2] = {nullptr, nullptr};
