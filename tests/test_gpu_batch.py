@@ -274,3 +274,40 @@ def test_config4_mix(gpu, oracle):
         assert errs[i, 0] == dd.max(), (specs[i], errs[i, 0], dd.max())
         assert abs(errs[i, 1] - dd.mean()) <= 1e-12 * dd.mean(), (specs[i], errs[i, 1], dd.mean())
         assert abs(errs[i, 2] - normwise) <= 1e-12 * normwise, (specs[i], errs[i, 2], normwise)
+
+
+def test_config4_mix_bf16(gpu, oracle):
+    # The same 4096 solves stored in BF16: every batched norm equals the
+    # one-grid-per-launch BF16 norm bit for bit, two solves (the largest
+    # grid and the largest grid R=4 at K=50) equal the oracle's BF16
+    # arithmetic on row bands, and every result is within 0.11 normwise of
+    # the same solve in FP64 (BF16 stores in_0 + k to within 2^-3 near 51:
+    # 0.073-0.075 at n=16384, K=50; the bound of the precision sweep).
+    import bench
+
+    specs = [(s[0], s[1], BF16) + tuple(s[3:]) for s in bench.config4_solves()]
+    big = max(range(len(specs)), key=lambda i: specs[i][3])
+    big_g4 = max((i for i, s in enumerate(specs) if s[0] == 1 and s[1] == 4), key=lambda i: specs[i][3] * specs[i][4])
+    print("config 4 in BF16: batch of 4096", flush=True)
+    b = Batch(specs)
+    b.run()
+    norms = b.wait()
+    outs = {i: b.output(i, 0) for i in (big, big_g4)}
+    b64 = Batch([(s[0], s[1], F64) + tuple(s[3:]) for s in specs])
+    b64.run()
+    b64.wait()
+    errs = b.errors(b64)
+    b64.close()
+    b.close()
+    print(f"config 4 in BF16: normwise error against FP64: max {errs[:, 2].max():.3e}, "
+          f"median {np.median(errs[:, 2]):.3e}", flush=True)
+    assert errs[:, 2].max() > 0 and np.all(errs[:, 2] <= 0.11), int(np.sum(errs[:, 2] > 0.11))
+    print("config 4 in BF16: one grid per launch", flush=True)
+    n16 = one_by_one(specs)
+    assert np.array_equal(norms, n16), int(np.sum(norms != n16))
+    for i in (big, big_g4):
+        p, r, d, n, k, init, seed = specs[i]
+        for a, bb in ((0, r + 6), (n // 2 - 4, n // 2 + 4), (n - r - 6, n)):
+            ref = oracle.band_lp(BF16, p, r, n, k, init, seed, a, bb)
+            assert np.array_equal(outs[i][a:bb], ref), (specs[i], a, bb)
+
