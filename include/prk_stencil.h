@@ -171,7 +171,8 @@ int prk_run(int pattern, int radius, int dtype, int n, long long pitch, int iter
  * variant: fused-sweep pipeline variant 0..5 (pipeline depth 1..3 x
  * temporal / non-temporal streams), 6 and 7 the register-window form of the
  * FP32/BF16 sweeps at R <= 3 (depth 2 and 3, non-temporal; other shapes run
- * them as 4 and 5), -1 = the tuned or default choice. */
+ * them as 4 and 5), 8 depth 4, non-temporal; -1 = the tuned or default
+ * choice. */
 int prk_set_rows_per_wave(int rows);
 int prk_set_sweep_variant(int variant);
 /* order: workgroup-to-tile order (0 column-major runs per XCD, 1 row-major,

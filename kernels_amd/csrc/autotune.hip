@@ -1,5 +1,5 @@
 // Autotuner for the fused sweep: for a (pattern, radius, dtype, n) it times
-// every candidate tiling on the device -- pipeline variant (depth 1..3 rows,
+// every candidate tiling on the device -- pipeline variant (depth 1..4 rows,
 // temporal or non-temporal streams, and for the FP32/BF16 forms at R <= 3
 // the register-window sweep), row-block height per wavefront and
 // resident workgroups per CU -- and records the fastest in a process-wide
@@ -26,7 +26,7 @@ typedef std::tuple<int, int, int, int> Key;  // pattern, radius, dtype, n
 std::mutex g_mu;
 std::map<Key, prk_tuning> g_table;
 
-const int kCandVariants[] = {1, 2, 4, 5, 6, 7};
+const int kCandVariants[] = {1, 2, 4, 5, 6, 7, 8};
 const int kCandWgs[] = {2, 3, 4};
 const int kCandRows[] = {4, 6, 8, 12, 16, 24, 32, 48, 64};
 // Row blocks beyond 32 only for R >= 4, where the 2R rows each block re-reads

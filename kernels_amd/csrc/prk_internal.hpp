@@ -31,8 +31,8 @@ struct SweepPlan {
 // Number of fused-sweep variants: 0..5 the LDS-ring sweep (pipeline depth
 // 1..3 x temporal / non-temporal), 6 and 7 the register-window sweep of the
 // FP32/BF16 forms at R <= 3 (depth 2 and 3, non-temporal; elsewhere they run
-// as 4 and 5).
-constexpr int kSweepVariants = 8;
+// as 4 and 5), 8 the LDS-ring sweep at depth 4, non-temporal.
+constexpr int kSweepVariants = 9;
 inline bool register_window_variant(int variant, int radius, int dtype)
 {
     return (variant == 6 || variant == 7) && dtype != F64 && radius <= 3;

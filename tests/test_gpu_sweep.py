@@ -213,7 +213,8 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
     ref, _ = oracle.run(pattern, radius, n, 32, iters, 1, 9)
     try:
         for v, rows, order, wgs in [(-1, 0, -1, -1), (0, 1, 0, 0), (3, 5, 2, 1), (5, 64, 1, 4), (2, 300, 1, 2),
-                                    (4, 8, 3, 3), (4, 8, 4, 3), (5, 4, 5, 2), (4, 3, 6, 3), (1, 7, 7, 4)]:
+                                    (4, 8, 3, 3), (4, 8, 4, 3), (5, 4, 5, 2), (4, 3, 6, 3), (1, 7, 7, 4),
+                                    (8, 8, 1, 3), (8, 5, 3, 2), (8, 300, 0, 4)]:
             K.check(L.prk_set_sweep_variant(v)); K.check(L.prk_set_rows_per_wave(rows))
             K.check(L.prk_set_sweep_order(order, wgs))
             b0, b1, out = (K.alloc_grid(n, K.F64) for _ in range(3))
@@ -229,14 +230,15 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
 @pytest.mark.parametrize("radius", [2, 3, 5])
 def test_grid_ring_sums_every_tiling(gpu, oracle, dtype, radius):
     # The FP32/BF16 grid ring sums in every pipeline variant of the LDS-ring
-    # sweep (1-3 rows in flight, cached or non-temporal stores) and, at
+    # sweep (1-4 rows in flight, cached or non-temporal stores) and, at
     # R <= 3, the register window, over several row heights, tile orders and
     # occupancies: bitwise the oracle's every time (BF16 grid R=5 reads its
     # windows whole, the others narrow).
     n, iters, pattern = 301, 3, 1
     L = K.lib()
     ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 21)
-    tilings = [(0, 1, 0, 0), (1, 5, 2, 1), (2, 64, 1, 4), (3, 7, 3, 3), (4, 8, 1, 2), (5, 12, 1, 4), (5, 300, 0, 1)]
+    tilings = [(0, 1, 0, 0), (1, 5, 2, 1), (2, 64, 1, 4), (3, 7, 3, 3), (4, 8, 1, 2), (5, 12, 1, 4), (5, 300, 0, 1),
+               (8, 8, 1, 3), (8, 6, 3, 2)]
     if radius <= 3:
         tilings += [(6, 6, 1, 4), (7, 13, 1, 2)]
     try:
@@ -376,7 +378,7 @@ def test_frame_columns_keep_their_bits(gpu, oracle, dtype):
     # matches the oracle (the frame of out feeds no interior point)
     L = K.lib()
     n, iters, seed = 616, 2, 8  # a multiple of every vector width: the whole-vector path
-    cases = [(0, 2, v) for v in (1, 4, 5)] + [(1, 3, 5), (0, 7, 4)]
+    cases = [(0, 2, v) for v in (1, 4, 5, 8)] + [(1, 3, 5), (1, 3, 8), (0, 7, 4)]
     if dtype != K.F64:
         cases += [(1, 2, 6), (0, 3, 7)]
     try:
