@@ -159,7 +159,7 @@ __device__ inline void sweep_tile(const BGrid &g, int k, int c0, int r0, int r1,
     void *nx = (k & 1) ? g.buf0 : g.buf1;
     // the grids are the library's own: out's frame may be rewritten with
     // the values it holds (FRAME_REWRITE)
-    wave_tile<DT, P, R, true, MODE_VEC, 2, 1, 1, false>(in, nx, g.out, g.pitch, g.n, whole_grid(g.n), c0, r0, r1, 1,
+    wave_tile<DT, P, R, true, MODE_VEC, 4, 1, 1, false>(in, nx, g.out, g.pitch, g.n, whole_grid(g.n), c0, r0, r1, 1,
                                                         ring, lane);
 }
 
