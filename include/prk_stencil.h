@@ -223,7 +223,7 @@ int prk_tuning_load(const char *path);
  * grid: initial values, iterations+1 fused sweeps, L1 norm.  A batch holds
  * many solves of any sizes, patterns, radii (1..8), iteration counts and
  * storage types; prk_batch_create allocates all their grids in one device
- * pool (pitch rounded up to 16 bytes; padding columns are scratch), cuts a
+ * pool (rows padded to 128 bytes; padding columns are scratch), cuts a
  * sliding-window schedule into rounds (one stage of each active solve per
  * round, one launch per round and (storage type, pattern, radius) class
  * over the tiles of its solves) and captures the whole run in a hipGraph,

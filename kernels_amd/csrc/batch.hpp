@@ -48,7 +48,7 @@ struct DDv {
 // scalar loads at the start of a tile.
 struct alignas(32) BGrid {
     void *buf0, *buf1, *out;
-    long long pitch;         // elements, 16-byte multiple; padding may be written
+    long long pitch;         // elements, 128-byte multiple (lib_pitch); padding may be written
     int n, pattern, radius;
     int iters;               // K: the solve runs K+1 sweeps
     int rows;                // rows per tile
