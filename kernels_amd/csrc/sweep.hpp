@@ -410,8 +410,8 @@ __device__ inline void pair_sum_from(const T *const (&rows)[2 * R + 1], float (&
 // The column sums VS and the differences D are formed once per column and
 // shared by the lane's V points.  Per point at R=3 that is 32 FP32
 // operations (BF16, V=8) or 35 (FP32, V=4) against the 42 of the paired
-// form (a subtraction and an fma per tap pair), at R=8 153 or 188 against
-// 272.  The BF16 grid sweeps and the large-R grids are bound by their VALU
+// form (a subtraction and an fma per tap pair), at R=8 at most 153 or 188
+// against 272 (column sums no point uses are never formed).  The BF16 grid sweeps and the large-R grids are bound by their VALU
 // issue and run faster for it (BF16 grid R=3 +2-3 %, R=5 +25 %, R=8 +46 %;
 // FP32 grid R=5 +14 %, R=8 +28 %); the HBM-bound ones do not notice.
 // ---------------------------------------------------------------------------
