@@ -3,8 +3,9 @@
 FP64 is bitwise: the kernels sum the taps in the generator's order with no
 contraction, exactly as the oracle (pinned to the reference's own generated
 operators in test_oracle.py).  FP32/BF16 storage forms are bitwise against
-the oracle's restatement of the same paired FP32 arithmetic, and their error
-against the FP64 oracle is bounded separately.
+the oracle's restatement of the same FP32 arithmetic (paired sums for the
+stars, ring sums for the grids), and their error against the FP64 oracle is
+bounded separately.
 """
 import numpy as np
 import pytest
