@@ -311,3 +311,25 @@ def test_config4_mix_bf16(gpu, oracle):
             ref = oracle.band_lp(BF16, p, r, n, k, init, seed, a, bb)
             assert np.array_equal(outs[i][a:bb], ref), (specs[i], a, bb)
 
+
+@pytest.mark.parametrize("rows", [0, 8, 17])
+def test_random_mix_bitwise(gpu, oracle, rows):
+    # A seeded random mix of 48 solves (pattern, radius 1..8, storage type,
+    # n 9..300, 0..3 iterations, either init), one batch per tile height:
+    # every out, final in and norm equals the oracle's (the norms to 1e-13).
+    rng = np.random.default_rng(777 + rows)
+    mix = []
+    for _ in range(48):
+        r = int(rng.integers(1, 9))
+        mix.append((int(rng.integers(0, 2)), r, int(rng.integers(0, 3)), int(rng.integers(max(9, 2 * r + 1), 301)),
+                    int(rng.integers(0, 4)), int(rng.integers(0, 2)), int(rng.integers(0, 1000))))
+    b = Batch(mix, rows_per_tile=rows)
+    b.run()
+    norms = b.wait()
+    for i, s in enumerate(mix):
+        out, fin, norm = ref_solve(oracle, s)
+        assert np.array_equal(b.output(i, 0).view(np.uint8), out.view(np.uint8)), s
+        assert np.array_equal(b.output(i, 1).view(np.uint8), fin.view(np.uint8)), s
+        assert abs(norms[i] - norm) <= 1e-13 * abs(norm) + 1e-300, (s, norms[i], norm)
+    b.close()
+
