@@ -79,3 +79,14 @@ def test_wrappers_validate_before_calling_the_library():
         K.run(K.STAR, 2, K.F64, 3, a, grid(16, pitch=18), b)
     with pytest.raises(ValueError, match="HIP device"):
         K.init(K.INIT_ANALYTIC, 0, K.F64, a, b)
+
+
+def test_default_pitch():
+    # rows padded to 128 bytes, rows of 64 KiB or more to 1 KiB
+    for dtype, es in ((K.F64, 8), (K.F32, 4), (K.BF16, 2)):
+        for n in (1, 7, 64, 100, 1000, 1025, 8191, 8192, 16384, 30000, 46340):
+            p = K.default_pitch(n, dtype)
+            align = 1024 if n * es >= 65536 else 128
+            assert p >= n and (p * es) % align == 0 and (p - n) * es < align, (dtype, n, p)
+    assert K.default_pitch(30000, K.F64) == 30080 and K.default_pitch(30000, K.BF16) == 30016
+    assert K.default_pitch(16384, K.F32) == 16384
