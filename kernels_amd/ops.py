@@ -109,8 +109,8 @@ def default_pitch(n: int, dtype: int) -> int:
 def alloc_grid(n: int, dtype: int = F64, device: str = "cuda", pitch: int | None = None,
                fill: float | None = None) -> torch.Tensor:
     """n x n grid stored with row stride `pitch` (default: default_pitch,
-    n rounded up to a 128-byte multiple).  Returns the (n, n) view; its
-    storage is (n, pitch)."""
+    rows rounded up to 128 bytes, or to 1 KiB for rows of 64 KiB or more).
+    Returns the (n, n) view; its storage is (n, pitch)."""
     if pitch is None:
         pitch = default_pitch(n, dtype)
     base = torch.empty((n, pitch), dtype=_TORCH[dtype], device=device)
