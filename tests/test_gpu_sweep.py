@@ -228,13 +228,13 @@ def test_autotune_and_tiling_do_not_change_results(gpu, oracle):
 
 
 @pytest.mark.parametrize("dtype", [F32, BF16])
-@pytest.mark.parametrize("radius", [2, 3, 5])
+@pytest.mark.parametrize("radius", [1, 2, 3, 4, 5, 7])
 def test_grid_ring_sums_every_tiling(gpu, oracle, dtype, radius):
     # The FP32/BF16 grid ring sums in every pipeline variant of the LDS-ring
     # sweep (1-4 rows in flight, cached or non-temporal stores) and, at
     # R <= 3, the register window, over several row heights, tile orders and
-    # occupancies: bitwise the oracle's every time (BF16 grid R=5 reads its
-    # windows whole, the others narrow).
+    # occupancies: bitwise the oracle's every time (BF16 grid R=5 and R=7
+    # read their windows whole, the others narrow).
     n, iters, pattern = 301, 3, 1
     L = K.lib()
     ref, _ = oracle.run_lp(dtype, pattern, radius, n, iters, 1, 21)
