@@ -289,7 +289,7 @@ def test_extended_radius_bitwise(gpu, oracle, pattern, radius, n):
 # Star radii that sweep from a register ring (FP64 R >= 6, FP32 R >= 5,
 # BF16 R >= 6): every row-block height from one row up, each pipeline depth,
 # aligned and unaligned pitch, bitwise against the oracle.
-RING = [(K.F64, 6), (K.F64, 8), (F32, 5), (BF16, 7)]
+RING = [(K.F64, 6), (K.F64, 8), (F32, 4), (F32, 5), (BF16, 7)]
 
 
 @pytest.mark.parametrize("dtype,radius", RING)
