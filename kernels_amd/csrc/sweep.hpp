@@ -519,9 +519,10 @@ __device__ inline void compute_points(const typename DTypeTraits<DT>::T *const (
 // ring held star R=5 and R=8 to 3 and 2 workgroups per CU).  The arithmetic
 // is that of compute_points, operand for operand.  Measured on MI355X at
 // n=16384 against the LDS ring: FP64 R=6/7/8 +2/+2/+4 %, FP32 R=4 +1.5 %,
-// FP32 R=5..8 +2..+6 %, BF16 R=6/7/8 +13/+15/+17 %; below those radii (BF16
-// R=4 and 5 included: -1.6 % and -2 % tuned) it lost 1-4 % (the unrolled ring
-// costs more than the occupancy gains), so they keep the LDS ring.
+// FP32 R=5..8 +2..+6 %, BF16 R=6/7/8 +13/+15/+17 %; below those radii (FP64
+// R=4 and 5: -3 % tuned; BF16 R=4 and 5: -1.6 % and -2 %) it lost 1-4 % (the
+// unrolled ring costs more than the occupancy gains), so they keep the LDS
+// ring.
 // ---------------------------------------------------------------------------
 template <int DT, int P, int R>
 constexpr bool star_reg_ring() { return P == STAR && R >= (DT == F32 ? 4 : 6); }
